@@ -1,0 +1,318 @@
+// Binned-SAH BVH build (Wald 2007, "On fast construction of SAH-based bounding
+// volume hierarchies") over triangle centroids, 32 bins on all three axes, then a
+// depth-first child-pair layout for the HIP traversal.
+//
+// Compared with recursiveBuild (src/geometry/bvh.cpp:114-307) this keeps the same
+// idea (binned SAH, small leaves, object-median fallback) but: bins all axes, uses
+// true surface area (the reference multiplies the three extents, bvh.cpp:334-337),
+// initialises every bound (F6), runs subtrees as OpenMP tasks, and caps the depth
+// so the traversal stack is bounded (F7). The tree depends only on the input, not
+// on the thread count: partitions are in place on disjoint ranges and node
+// numbering is a depth-first walk after the build.
+#include "bvh.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace prt {
+namespace {
+
+constexpr int kBins = 32;
+constexpr uint32_t kTaskGrain = 32768;
+
+struct Prim {
+    float lo[3], hi[3], c[3];
+};
+struct BNode {
+    float lo[3], hi[3];
+    int32_t child[2];
+    uint32_t start, count, depth;
+};
+
+inline void box_init(float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = FLT_MAX;
+        hi[k] = -FLT_MAX;
+    }
+}
+inline void box_grow(float lo[3], float hi[3], const float a[3], const float b[3]) {
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], a[k]);
+        hi[k] = std::max(hi[k], b[k]);
+    }
+}
+inline float box_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0.f || dy < 0.f || dz < 0.f) return 0.f;
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Ctx {
+    const std::vector<Prim>* prims;
+    uint32_t* idx;
+    BNode* nodes;
+    std::atomic<uint32_t> next{0};
+    BvhParams p;
+};
+
+void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth) {
+    const std::vector<Prim>& P = *c.prims;
+    BNode& nd = c.nodes[me];
+    nd.start = start;
+    nd.count = end - start;
+    nd.child[0] = nd.child[1] = -1;
+    nd.depth = depth;
+    float clo[3], chi[3];
+    box_init(nd.lo, nd.hi);
+    box_init(clo, chi);
+    for (uint32_t i = start; i < end; ++i) {
+        const Prim& q = P[c.idx[i]];
+        box_grow(nd.lo, nd.hi, q.lo, q.hi);
+        box_grow(clo, chi, q.c, q.c);
+    }
+    const uint32_t n = end - start;
+    if (n == 1) return;
+
+    uint32_t mid = start;
+    bool have_split = false;
+    if (depth < uint32_t(c.p.sah_depth_cap)) {
+        int best_axis = -1, best_bin = -1;
+        float best_cost = FLT_MAX;
+        for (int ax = 0; ax < 3; ++ax) {
+            float ext = chi[ax] - clo[ax];
+            if (!(ext > 0.f)) continue;
+            float scale = float(kBins) / ext;
+            uint32_t cnt[kBins] = {0};
+            float blo[kBins][3], bhi[kBins][3];
+            for (int b = 0; b < kBins; ++b) box_init(blo[b], bhi[b]);
+            for (uint32_t i = start; i < end; ++i) {
+                const Prim& q = P[c.idx[i]];
+                int b = std::min(kBins - 1, int((q.c[ax] - clo[ax]) * scale));
+                cnt[b]++;
+                box_grow(blo[b], bhi[b], q.lo, q.hi);
+            }
+            float rarea[kBins];
+            uint32_t rcnt[kBins];
+            float lo[3], hi[3];
+            box_init(lo, hi);
+            uint32_t acc = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                box_grow(lo, hi, blo[b], bhi[b]);
+                acc += cnt[b];
+                rarea[b] = box_area(lo, hi);
+                rcnt[b] = acc;
+            }
+            box_init(lo, hi);
+            acc = 0;
+            for (int b = 0; b < kBins - 1; ++b) {
+                box_grow(lo, hi, blo[b], bhi[b]);
+                acc += cnt[b];
+                if (acc == 0 || rcnt[b + 1] == 0) continue;
+                float cost = box_area(lo, hi) * float(acc) + rarea[b + 1] * float(rcnt[b + 1]);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = ax;
+                    best_bin = b;
+                }
+            }
+        }
+        if (best_axis >= 0) {
+            float parea = box_area(nd.lo, nd.hi);
+            float split_cost = 1.0f + (parea > 0.f ? best_cost / parea : 0.f);
+            float leaf_cost = float(n);
+            if (n <= uint32_t(c.p.leaf_size) && leaf_cost <= split_cost) return;  // SAH leaf
+            const int ax = best_axis;
+            const float scale = float(kBins) / (chi[ax] - clo[ax]);
+            const float cmin = clo[ax];
+            uint32_t* it = std::partition(c.idx + start, c.idx + end, [&](uint32_t t) {
+                return std::min(kBins - 1, int((P[t].c[ax] - cmin) * scale)) <= best_bin;
+            });
+            mid = uint32_t(it - c.idx);
+            have_split = mid > start && mid < end;
+        } else if (n <= uint32_t(c.p.leaf_size)) {
+            return;  // all centroids coincide: one leaf
+        }
+    } else if (n <= uint32_t(c.p.leaf_size)) {
+        return;
+    }
+    if (!have_split) {
+        // Object median on the axis of largest centroid extent (index middle if none).
+        mid = start + n / 2;
+        float d[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+        int ax = (d[0] >= d[1] && d[0] >= d[2]) ? 0 : (d[1] >= d[2] ? 1 : 2);
+        if (d[ax] > 0.f)
+            std::nth_element(c.idx + start, c.idx + mid, c.idx + end, [&](uint32_t a, uint32_t b) {
+                if (P[a].c[ax] != P[b].c[ax]) return P[a].c[ax] < P[b].c[ax];
+                return a < b;
+            });
+    }
+    uint32_t l = c.next.fetch_add(2);
+    uint32_t r = l + 1;
+    nd.child[0] = int32_t(l);
+    nd.child[1] = int32_t(r);
+    if (n > kTaskGrain) {
+#pragma omp task default(none) firstprivate(l, start, mid, depth) shared(c)
+        build_rec(c, l, start, mid, depth + 1);
+        build_rec(c, r, mid, end, depth + 1);
+#pragma omp taskwait
+    } else {
+        build_rec(c, l, start, mid, depth + 1);
+        build_rec(c, r, mid, end, depth + 1);
+    }
+}
+
+struct Emitter {
+    const BNode* bn;
+    const float (*pbox)[6];
+    std::vector<float>& out;
+    uint32_t next = 1;
+    uint32_t leaves = 0, max_depth = 0, max_leaf = 0;
+    double sah = 0.0;
+    double root_area = 1.0;
+
+    int32_t child_word(int32_t b, uint32_t depth) {
+        const BNode& c = bn[b];
+        if (c.child[0] < 0) {
+            leaves++;
+            max_leaf = std::max(max_leaf, c.count);
+            sah += double(box_area(pbox[b], pbox[b] + 3)) / root_area * double(c.count);
+            return leaf_word(c.start, c.count);
+        }
+        uint32_t k = next++;
+        emit(b, k, depth + 1);
+        return int32_t(k);
+    }
+    void write_box(uint32_t k, int slot, const float* bx) {
+        float* nd = &out[size_t(k) * 16];
+        // bx = lo[3], hi[3]
+        nd[slot * 4 + 0] = bx[0];
+        nd[slot * 4 + 1] = bx[3];
+        nd[slot * 4 + 2] = bx[1];
+        nd[slot * 4 + 3] = bx[4];
+        nd[8 + slot * 2 + 0] = bx[2];
+        nd[8 + slot * 2 + 1] = bx[5];
+    }
+    void emit(int32_t b, uint32_t k, uint32_t depth) {
+        max_depth = std::max(max_depth, depth);
+        sah += double(box_area(pbox[b], pbox[b] + 3)) / root_area;
+        const BNode& nd = bn[b];
+        int32_t w0 = child_word(nd.child[0], depth);
+        int32_t w1 = child_word(nd.child[1], depth);
+        write_box(k, 0, pbox[nd.child[0]]);
+        write_box(k, 1, pbox[nd.child[1]]);
+        float* o = &out[size_t(k) * 16];
+        std::memcpy(&o[12], &w0, 4);
+        std::memcpy(&o[13], &w1, 4);
+        o[14] = 0.f;
+        o[15] = 0.f;
+    }
+};
+
+}  // namespace
+
+void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout& out) {
+    if (n == 0) throw std::invalid_argument("BVH of an empty scene");
+    if (n >= kMaxRecords) throw std::invalid_argument("more than 2^28 triangles");
+    if (params.leaf_size < 1 || params.leaf_size > kMaxLeaf) throw std::invalid_argument("leaf size out of range");
+    std::vector<Prim> prims(n);
+    float abs_max = 1.f;
+#pragma omp parallel for reduction(max : abs_max) schedule(static)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+        const float* p = pos + size_t(i) * 9;
+        Prim& q = prims[size_t(i)];
+        for (int k = 0; k < 3; ++k) {
+            q.lo[k] = std::min(p[k], std::min(p[3 + k], p[6 + k]));
+            q.hi[k] = std::max(p[k], std::max(p[3 + k], p[6 + k]));
+            q.c[k] = (q.lo[k] + q.hi[k]) * 0.5f;
+            abs_max = std::max(abs_max, std::max(std::fabs(q.lo[k]), std::fabs(q.hi[k])));
+        }
+    }
+    std::vector<uint32_t> idx(n);
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    std::vector<BNode> bn(size_t(2) * n + 1);
+    Ctx c;
+    c.prims = &prims;
+    c.idx = idx.data();
+    c.nodes = bn.data();
+    c.p = params;
+    c.next = 1;
+#pragma omp parallel
+#pragma omp single
+    build_rec(c, 0, 0, n, 0);
+    const uint32_t total = c.next.load();
+
+    // Padded boxes, bottom-up: a leaf box grows by pad_rel of its largest extent
+    // plus 2^-16 of the largest scene coordinate, an inner box is the union of its
+    // children's padded boxes. The padding keeps box rejection conservative
+    // with respect to the intersection test.
+    std::vector<float> pb(size_t(total) * 6);
+    const float abs_pad = abs_max * 0x1p-16f;
+    for (int64_t i = int64_t(total) - 1; i >= 0; --i) {
+        const BNode& nd = bn[size_t(i)];
+        float* b = &pb[size_t(i) * 6];
+        if (nd.child[0] < 0) {
+            float e = std::max(nd.hi[0] - nd.lo[0], std::max(nd.hi[1] - nd.lo[1], nd.hi[2] - nd.lo[2]));
+            float pad = e * params.pad_rel + abs_pad;
+            for (int k = 0; k < 3; ++k) {
+                b[k] = nd.lo[k] - pad;
+                b[3 + k] = nd.hi[k] + pad;
+            }
+        } else {
+            const float* a = &pb[size_t(nd.child[0]) * 6];
+            const float* d = &pb[size_t(nd.child[1]) * 6];
+            for (int k = 0; k < 3; ++k) {
+                b[k] = std::min(a[k], d[k]);
+                b[3 + k] = std::max(a[3 + k], d[3 + k]);
+            }
+        }
+    }
+    // Children of a build node always have larger indices, so the loop above
+    // visits children first.
+
+    uint32_t inner = 0;
+    for (uint32_t i = 0; i < total; ++i) inner += bn[i].child[0] >= 0 ? 1u : 0u;
+    out = BvhLayout();
+    if (inner == 0) {
+        // Whole scene in one leaf: root pairs it with an empty child whose box is inverted.
+        out.nodes.assign(16, 0.f);
+        float* o = out.nodes.data();
+        const float* b = &pb[0];
+        float empty[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        auto put = [&](int slot, const float* bx) {
+            o[slot * 4 + 0] = bx[0];
+            o[slot * 4 + 1] = bx[3];
+            o[slot * 4 + 2] = bx[1];
+            o[slot * 4 + 3] = bx[4];
+            o[8 + slot * 2 + 0] = bx[2];
+            o[8 + slot * 2 + 1] = bx[5];
+        };
+        put(0, b);
+        put(1, empty);
+        int32_t w0 = leaf_word(0, n), w1 = leaf_word(0, 1);
+        std::memcpy(&o[12], &w0, 4);
+        std::memcpy(&o[13], &w1, 4);
+        out.num_nodes = 1;
+        out.num_leaves = 1;
+        out.max_leaf = n;
+        out.sah_cost = float(n);
+    } else {
+        out.nodes.assign(size_t(inner) * 16, 0.f);
+        Emitter em{bn.data(), reinterpret_cast<const float(*)[6]>(pb.data()), out.nodes};
+        em.root_area = std::max(1e-30, double(box_area(&pb[0], &pb[3])));
+        em.emit(0, 0, 0);
+        out.num_nodes = em.next;
+        out.num_leaves = em.leaves;
+        out.max_depth = em.max_depth;
+        out.max_leaf = em.max_leaf;
+        out.sah_cost = float(em.sah);
+    }
+    out.order = std::move(idx);
+}
+
+}  // namespace prt

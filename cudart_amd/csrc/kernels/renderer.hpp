@@ -1,0 +1,43 @@
+// prt_renderer: one image on one device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../host/scene.hpp"
+#include "kernel_api.hpp"
+
+namespace prt {
+struct WavefrontState;
+}
+
+struct prt_renderer {
+    prt_scene* scene = nullptr;
+    prt_camera cam{};
+    uint32_t w = 0, h = 0, spp = 0;
+    prt_render_config cfg{};
+    prt::SceneArgs sc{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float4* accum = nullptr;               // w*h running sums
+    unsigned long long* counters = nullptr;
+    uint8_t* gamma_lut = nullptr;
+    int* spill = nullptr;
+    uint32_t spill_threads = 0;
+    uint32_t tiles_x = 0, tiles = 0;
+    uint32_t samples_done = 0;
+    uint64_t samples_total = 0;
+    double last_ms = 0.0;
+    prt::WavefrontState* wf = nullptr;
+};
+
+namespace prt {
+int hip_fail(hipError_t e, const char* what);
+int device_scene(prt_scene* s, int device, SceneArgs& out);
+CamArgs cam_args(const prt_camera& c);
+int set_device(int device);
+int ensure_spill(prt_renderer* r, uint32_t threads);
+int render_wavefront(prt_renderer* r, uint32_t n);  // wavefront.hip
+void release_wavefront(prt_renderer* r);            // wavefront.hip
+}  // namespace prt
