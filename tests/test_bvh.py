@@ -1,0 +1,205 @@
+"""BVH invariants (product child-pair layout and the oracle's reference-shaped
+tree) and closest-hit equality with an O(N) brute-force scan."""
+import numpy as np
+import pytest
+
+import cudart_amd as ca
+import oracle as O
+from cudart_amd import workloads as W
+
+
+def _tri_bounds(pos):
+    return pos.min(axis=1), pos.max(axis=1)
+
+
+def _check_product_layout(mesh):
+    a = mesh.arrays()
+    s = ca.Scene(mesh)
+    info = s.bvh_info()
+    nodes, recs, order = s.export_bvh()
+    n = len(a.positions)
+    assert sorted(order.tolist()) == list(range(n))
+    # records: v1, e1 = v2 - v1, e2 = v3 - v1, original id and material
+    p = a.positions[order]
+    np.testing.assert_array_equal(recs[:, 0:3], p[:, 0])
+    np.testing.assert_array_equal(recs[:, 4:7], p[:, 1] - p[:, 0])
+    np.testing.assert_array_equal(recs[:, 8:11], p[:, 2] - p[:, 0])
+    assert np.array_equal(recs[:, 3].view(np.uint32), order)
+    assert np.array_equal(recs[:, 7].view(np.uint32), a.material_ids[order])
+    words = nodes[:, 12:14].view(np.int32)
+    lo, hi = _tri_bounds(a.positions)
+    covered = np.zeros(n, np.int32)
+    seen_nodes = np.zeros(len(nodes), np.int32)
+    depth_max = 0
+    stack = [(0, np.full(3, -np.inf), np.full(3, np.inf), 0)]
+    while stack:
+        k, plo, phi, depth = stack.pop()
+        seen_nodes[k] += 1
+        depth_max = max(depth_max, depth)
+        nd = nodes[k]
+        for c in range(2):
+            clo = np.array([nd[c * 4 + 0], nd[c * 4 + 2], nd[8 + c * 2]])
+            chi = np.array([nd[c * 4 + 1], nd[c * 4 + 3], nd[8 + c * 2 + 1]])
+            w = int(words[k, c])
+            if clo[0] > chi[0]:  # inverted box: the empty partner of a one-leaf root
+                continue
+            assert (clo >= plo).all() and (chi <= phi).all(), "child box outside its parent's"
+            if w >= 0:
+                stack.append((w, clo, chi, depth + 1))
+            else:
+                u = w & 0xFFFFFFFF
+                first, count = u & 0x0FFFFFFF, ((u >> 28) & 7) + 1
+                assert count <= 8
+                ids = order[first:first + count]
+                assert (lo[ids] >= clo).all() and (hi[ids] <= chi).all(), "leaf box does not contain a triangle"
+                assert (lo[ids] > clo).all() and (hi[ids] < chi).all(), "leaf box is not padded"
+                covered[first:first + count] += 1
+    assert (covered == 1).all(), "every record in exactly one leaf"
+    assert (seen_nodes == 1).all(), "every node reached once"
+    assert depth_max == info.max_depth
+    return info
+
+
+def test_product_bvh_cornell():
+    _check_product_layout(W.cornell().mesh)
+
+
+def test_product_bvh_procedural_10k():
+    info = _check_product_layout(ca.Mesh.generate("mixed", "tris=10000", 3))
+    assert info.max_leaf_size <= 4
+
+
+def test_product_bvh_single_leaf_and_degenerate():
+    from tests.helpers import OwnedDesc, material, quad
+    cam = ca.camera_look_at((0, 1, 3), (0, 0, 0), (0, 1, 0), 40, 8, 8)
+    p, n = quad([-1, 0, -1], [-1, 0, 1], [1, 0, 1], [1, 0, -1])
+    od = OwnedDesc(p, n, [0, 1], [material(0), material(5, ka=(1, 1, 1))], cam)
+    s = ca.Scene(od.desc)
+    info = s.bvh_info()
+    assert info.num_nodes == 1 and info.num_leaves == 1
+    # 40 copies of the same triangle: centroids coincide, splits fall back to index halves
+    P = np.repeat(p[:1], 40, axis=0)
+    N = np.repeat(n[:1], 40, axis=0)
+    od2 = OwnedDesc(P, N, [0] * 39 + [1], [material(0), material(5, ka=(1, 1, 1))], cam)
+    s2 = ca.Scene(od2.desc)
+    assert s2.bvh_info().max_leaf_size <= 4
+
+
+def _check_oracle_tree(mesh):
+    a = mesh.arrays()
+    s = O.OracleScene(mesh.desc())
+    b, m, order = s.export_bvh()
+    n = len(a.positions)
+    assert sorted(order.tolist()) == list(range(n))
+    lo, hi = _tri_bounds(a.positions[order])
+    covered = np.zeros(n, np.int32)
+
+    def walk(i, plo, phi):
+        nlo, nhi = b[i, :3], b[i, 3:]
+        assert (nlo >= plo).all() and (nhi <= phi).all()
+        off, cnt, axis = m[i]
+        if cnt > 0:
+            assert (lo[off:off + cnt] >= nlo).all() and (hi[off:off + cnt] <= nhi).all()
+            if cnt > 3:  # only when every centroid in the node coincides
+                c = (lo[off:off + cnt] + hi[off:off + cnt]) * 0.5
+                assert (np.ptp(c, axis=0) == 0).any()
+            covered[off:off + cnt] += 1
+            return
+        walk(i + 1, nlo, nhi)
+        walk(off, nlo, nhi)
+
+    walk(0, np.full(3, -np.inf), np.full(3, np.inf))
+    assert (covered == 1).all()
+
+
+def test_oracle_bvh_invariants():
+    _check_oracle_tree(W.cornell().mesh)
+    _check_oracle_tree(ca.Mesh.generate("mixed", "tris=10000", 3))
+
+
+def _random_rays(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    c = (lo + hi) / 2
+    ext = (hi - lo).max()
+    o = c + (rng.random((n, 3)) - 0.5) * ext * 1.5
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["cornell", "mixed10k"])
+def test_oracle_bvh_matches_brute_force(which):
+    mesh = W.cornell().mesh if which == "cornell" else ca.Mesh.generate("mixed", "tris=10000", 3)
+    a = mesh.arrays()
+    lo, hi = a.positions.reshape(-1, 3).min(0), a.positions.reshape(-1, 3).max(0)
+    o, d = _random_rays(10000, lo, hi, 5)
+    # axis-aligned and grazing rays along the walls of the box
+    g = np.random.default_rng(6)
+    o2 = (lo + g.random((2000, 3)) * (hi - lo)).astype(np.float32)
+    d2 = np.zeros((2000, 3), np.float32)
+    d2[np.arange(2000), g.integers(0, 3, 2000)] = g.choice([-1.0, 1.0], 2000)
+    d2 += (g.normal(size=(2000, 3)) * 1e-4).astype(np.float32)
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    # rays aimed at random points of random triangles
+    k = g.integers(0, len(a.positions), 4000)
+    bary = g.dirichlet([1, 1, 1], 4000)
+    tgt = np.einsum("nk,nkj->nj", bary, a.positions[k])
+    o3 = (tgt + g.normal(size=(4000, 3)) * (hi - lo).max() * 0.3).astype(np.float32)
+    d3 = tgt - o3
+    d3 = (d3 / np.linalg.norm(d3, axis=1, keepdims=True)).astype(np.float32)
+    o = np.concatenate([o, o2, o3])
+    d = np.concatenate([d, d2.astype(np.float32), d3])
+    s = O.OracleScene(mesh.desc())
+    t1, tt1, u1, v1 = s.trace(o, d)
+    t2, tt2, u2, v2 = s.trace(o, d, brute=True)
+    assert (t1 >= 0).mean() > 0.3
+    assert np.array_equal(t1, t2)
+    assert np.array_equal(tt1.view(np.uint32), tt2.view(np.uint32))
+    assert np.array_equal(u1, u2) and np.array_equal(v1, v2)
+
+
+def test_oracle_first_hit_matches_numpy_moller_trumbore():
+    """Closest hits from an independent vectorised fp32 Moller-Trumbore (same
+    operation order; numpy rounds each operation) over all triangles."""
+    wl = W.cornell(24, 16)
+    a = wl.mesh.arrays()
+    s = O.OracleScene(wl.mesh.desc())
+    tri, mat, t = s.first_hit(wl.camera, 24, 16, 2, O.cfg())
+    f = np.float32
+    v1 = a.positions[:, 0]
+    e1 = a.positions[:, 1] - v1
+    e2 = a.positions[:, 2] - v1
+
+    def cross(x, y):
+        return np.stack([x[..., 1] * y[..., 2] - x[..., 2] * y[..., 1], x[..., 2] * y[..., 0] - x[..., 0] * y[..., 2],
+                         x[..., 0] * y[..., 1] - x[..., 1] * y[..., 0]], -1)
+
+    def dot(x, y):
+        return (x[..., 0] * y[..., 0] + x[..., 1] * y[..., 1]) + x[..., 2] * y[..., 2]
+
+    assert (tri >= 0).mean() > 0.5 and np.array_equal(tri >= 0, mat >= 0)
+    rng = np.random.default_rng(0)
+    o = np.repeat(np.array([wl.camera.eye], f), 64, 0)
+    dirs = rng.normal(size=(64, 3)).astype(f)
+    dirs[:, 0] = -np.abs(dirs[:, 0]) - 1.0
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True).astype(f)
+    dirs = dirs.astype(f)
+    ids, ts, _, _ = s.trace(o, dirs)
+    for i in range(64):
+        d = dirs[i][None, :]
+        P = cross(d, e2)
+        det = dot(e1, P)
+        ok = np.abs(det) >= f(1e-5)
+        inv = (f(1) / np.where(ok, det, f(1))).astype(f)
+        T = (o[i][None, :] - v1).astype(f)
+        u = (dot(T, P) * inv).astype(f)
+        Q = cross(T, e1)
+        v = (dot(d, Q) * inv).astype(f)
+        tt = (dot(e2, Q) * inv).astype(f)
+        ok &= (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (tt > f(1e-5))
+        if not ok.any():
+            assert ids[i] == -1
+            continue
+        best = np.min(tt[ok])
+        cand = np.nonzero(ok & (tt == best))[0]
+        assert ids[i] == cand.min() and ts[i] == best
