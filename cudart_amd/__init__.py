@@ -287,6 +287,50 @@ class ParallelRenderer:
             pass
 
 
+@dataclass
+class BatchJob:
+    scene: Scene
+    camera: Camera
+    spp: int
+    view_id: int = 0
+
+
+def render_batch(jobs: list, config: RenderConfig | None = None, tune: "_capi.Tune | None" = None, **cfg):
+    """Renders every job (views of possibly different scenes, sizes and spp) in one
+    persistent wavefront pass. Returns (list of (h, w, 3) mean-radiance images, Stats)."""
+    c = config if config is not None else default_config(**cfg)
+    arr = (_capi.Job * len(jobs))()
+    outs = []
+    for i, j in enumerate(jobs):
+        img = np.zeros((j.camera.ypixels, j.camera.xpixels, 3), np.float32)
+        outs.append(img)
+        arr[i].scene = j.scene._h
+        arr[i].camera = j.camera
+        arr[i].spp = j.spp
+        arr[i].view_id = j.view_id
+        arr[i].out_rgb = _capi.fptr(img)
+    st = Stats()
+    check(lib().prt_render_batch(arr, len(jobs), C.byref(c), C.byref(tune) if tune is not None else None,
+                                 C.byref(st)))
+    return outs, st
+
+
+def autotune(jobs: list, config: RenderConfig | None = None, **cfg):
+    """Times queue sizes and persistent occupancy on a 2-spp trial of `jobs`; returns (Tune, ms)."""
+    c = config if config is not None else default_config(**cfg)
+    arr = (_capi.Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i].scene = j.scene._h
+        arr[i].camera = j.camera
+        arr[i].spp = j.spp
+        arr[i].view_id = j.view_id
+        arr[i].out_rgb = None
+    t = _capi.Tune()
+    ms = C.c_double()
+    check(lib().prt_autotune(arr, len(jobs), C.byref(c), C.byref(t), C.byref(ms)))
+    return t, ms.value
+
+
 def write_png(path, rgba: np.ndarray) -> None:
     a = np.ascontiguousarray(rgba, np.uint8)
     check(lib().prt_write_png(str(path).encode(), a.shape[1], a.shape[0], _capi.bptr(a)))

@@ -129,21 +129,24 @@ struct Counts {
 // ---- traversal -----------------------------------------------------------
 // Short stack in LDS (kStackLds entries per lane, lane-interleaved so any mix of
 // depths is bank-conflict free), continued in a global spill area.
-struct Stack {
+template <int LDS>
+struct StackT {
+    static constexpr int kLds = LDS;
     int* lds;              // this lane's column: lds[k * kBlock]
     int* spill;            // this thread's column in the spill area
     uint32_t spill_stride; // threads sharing the spill area
     int sp;
     __device__ __forceinline__ void push(int v) {
-        if (sp < kStackLds) lds[sp * kBlock] = v;
-        else spill[size_t(sp - kStackLds) * spill_stride] = v;
+        if (sp < LDS) lds[sp * kBlock] = v;
+        else spill[size_t(sp - LDS) * spill_stride] = v;
         ++sp;
     }
     __device__ __forceinline__ int pop() {
         --sp;
-        return sp < kStackLds ? lds[sp * kBlock] : spill[size_t(sp - kStackLds) * spill_stride];
+        return sp < LDS ? lds[sp * kBlock] : spill[size_t(sp - LDS) * spill_stride];
     }
 };
+using Stack = StackT<kStackLds>;
 
 struct HitRec {
     float t, u, v;
@@ -173,8 +176,8 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
 // closest hit is the lexicographic minimum of (t, original id), independent of
 // tree shape and visiting order. Returns true when best changed (closest) or
 // when an occluder was found (any).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float tmin, HitRec& best, Stack& st,
+template <bool ANY, bool COUNT, class StackType>
+__device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float tmin, HitRec& best, StackType& st,
                                          Counts& cnt) {
     bool found = false;
     if (d.x == 0.f && d.y == 0.f && d.z == 0.f) return false;  // det == 0 for every triangle

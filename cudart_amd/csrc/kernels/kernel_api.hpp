@@ -9,7 +9,8 @@
 namespace prt {
 
 constexpr int kBlock = 64;       // one wave64 per workgroup: an 8x8 pixel tile
-constexpr int kStackLds = 32;    // traversal stack entries per lane in LDS
+constexpr int kStackLds = 32;    // traversal stack entries per lane in LDS (megakernel)
+constexpr int kStackLdsQueue = 16;  // LDS entries of the queue-based traversal kernels
 constexpr int kStackMax = 128;   // total entries (the rest in the global spill area)
 
 // Device view of a built scene. Layouts:

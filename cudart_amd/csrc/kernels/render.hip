@@ -5,7 +5,7 @@
 // sample order, the order SequentialRenderer::renderOneSamplePerPixel
 // (src/renderer/sequential_renderer.cu:91-116) and renderKernel
 // (src/renderer/parallel_renderer.cu:318-342) accumulate in.
-#include "device_common.hpp"
+#include "path.hpp"
 
 namespace prt {
 namespace dev {
@@ -17,182 +17,36 @@ __device__ V3 sample_path(const SceneArgs& sc, const CamArgs& cam, const PathArg
     Rng rng{pa.seed, pa.view, uint32_t(y) * uint32_t(cam.xpixels) + uint32_t(x), sample};
     V3 ro, rd;
     camera_ray(cam, x, y, rng.block(0), ro, rd);
-
-    V3 color = v3(0.f, 0.f, 0.f), mask = v3(1.f, 1.f, 1.f);
-    V3 inputDirection = v3(0.f, 0.f, 0.f);  // SurfaceInteraction() is zeroed; stale across bounces (Q4)
-    int previousBsdf = 0;
-    const V3 sky = v3(0.0f, 0.025f, 0.05f);
-    const float invLA = sc.inv_lights_area;
-
-    for (uint32_t b = 0; b < pa.max_depth; ++b) {
+    PathState ps;
+    path_init(ps, ro, rd);
+    while (true) {
         if (COUNT) {
-            if (b == 0) cnt.primary++; else cnt.extension++;
+            if (ps.b == 0) cnt.primary++; else cnt.extension++;
         }
         HitRec h;
         h.t = kFltMax;
         h.id = 0;
         h.rec = 0;
         h.u = h.v = 0.f;
-        if (!traverse<false, COUNT>(sc, ro, rd, kEps, h, st, cnt)) {
-            color = color + mask * sky;
+        if (!traverse<false, COUNT>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) {
+            path_miss(ps);
             break;
         }
-        if (COUNT) cnt.hits++;
-        const float4* tp = sc.tris + size_t(h.rec) * 3;
-        const uint32_t m = __float_as_uint(tp[1].w);
-        const float4* mp = sc.mats + size_t(m) * 4;
-        const float4 mkd = mp[0], mka = mp[1], mks = mp[2], maux = mp[3];
-        const int bsdf0 = __float_as_int(mkd.w);
-        if (b == 0 || previousBsdf == 1 /*SPECULAR*/ || previousBsdf == 4 /*REFRACTIVE*/)
-            color = color + mask * v3(mka) * invLA;
-
-        const float4* shp = sc.shade + size_t(h.rec) * 4;
-        const float4 s0 = shp[0], s1 = shp[1], s2 = shp[2];
-        const V3 normal = shading_normal(s0, s1, s2, h.u, h.v);
-        const V3 position = ro + rd * h.t + normal * kEps2;
-        const V3 outputDirection = normalize(rd);
-        int bsdf = bsdf0;
-        uint4 ra = make_uint4(0, 0, 0, 0);
-        const bool needA = bsdf == 0 || bsdf == 2 || bsdf == 4;
-        if (needA) ra = rng.block(1u + 2u * b);
-
-        if (bsdf == 2 /*DIFFSPEC*/) {
-            float p = maux.y;
-            if (u01(ra.x) < p) {
-                bsdf = 0;
-                mask = mask * (1.0f / p);
-            } else {
-                bsdf = 1;
-                mask = mask * (1.0f / (1.0f - p));
-            }
+        bool want_shadow;
+        ShadowReq sr;
+        const bool more = shade_vertex<COUNT>(sc, pa, rng, ps, h, &want_shadow, sr, cnt);
+        if (want_shadow) {
+            HitRec occ;
+            occ.t = sr.tL;
+            occ.id = sr.lid;
+            occ.rec = 0;
+            occ.u = occ.v = 0.f;
+            const bool occluded = traverse<true, COUNT>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
+            ps.color = add_direct(ps.color, sr, !occluded);
         }
-        if (bsdf == 4 /*REFRACTIVE*/) {
-            // getFresnelReflectance (renderer.cu:464-493) with its quirks (Q5).
-            V3 tdir = v3(0.f, 0.f, 0.f);
-            V3 n = normal;
-            float cosi = dot(outputDirection, normal);
-            float etai = 1.f, etat = maux.x;
-            if (cosi < 0.0f) {
-                cosi = -cosi;
-            } else {
-                float tmp = etai;
-                etai = etat;
-                etat = tmp;
-                n = normal * -1.0f;
-            }
-            float eta = etai / etat;
-            float k = 1.f - eta * eta * (1.f - cosi * cosi);
-            float Re, Tr;
-            if (k < 0.f) {
-                Re = 1.0f;
-                Tr = 0.0f;
-            } else {
-                tdir = outputDirection * eta + n * (eta * cosi - sqrtf(k));
-                float R0 = (etai - etat) * (etai - etat) / (etat + etai) * (etat + etai);
-                float c = 1.f - dot(tdir, normal);
-                Re = R0 + (1.f - R0) * c * c * c * c * c;
-                Tr = 1.f - Re;
-            }
-            if (Re == 1.0f) {
-                bsdf = 1;
-            } else {
-                float P = 0.25f + .5f * Re;
-                float RP = Re / P;
-                float TP = Tr / (1.f - P);
-                if (u01(ra.x) > 0.25f) {
-                    inputDirection = tdir;
-                    mask = mask * v3(mks) * TP;
-                } else {
-                    mask = mask * RP;
-                    bsdf = 1;
-                }
-            }
-        }
-        uint4 rb = make_uint4(0, 0, 0, 0);
-        const bool needB = bsdf == 0 || b >= pa.rr_start;
-        if (needB) rb = rng.block(2u + 2u * b);
-        if (bsdf == 0 /*DIFFUSE*/) {
-            // sampleDiffuseBSDF (renderer.cu:375-402): pdf 0.5, weight kd * cos (Q1).
-            float r1 = float(2 * M_PI * double(u01(ra.y)));
-            float r2 = u01(ra.z);
-            float r2sq = sqrtf(r2);
-            V3 w = normal;
-            V3 a = fabsf(w.x) > 0.1f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
-            V3 uu = normalize(cross(a, w));
-            V3 vv = cross(w, uu);
-            float sr, cr;
-            own_sincosf(r1, &sr, &cr);
-            inputDirection = normalize(uu * cr * r2sq + vv * sr * r2sq + w * sqrtf(1.f - r2));
-            float cw = dot(inputDirection, normal);
-            V3 kd = v3(mkd);
-            const int tex = __float_as_int(mka.w);
-            if (tex >= 0) {
-                const float4 s3 = shp[3];
-                float bu = h.u, bv = h.v, bw = 1.f - bu - bv;
-                // uv1 = (s2.y, s2.z), uv2 = (s2.w, s3.x), uv3 = (s3.y, s3.z)
-                float tu = s2.y * bw + s2.w * bu + s3.y * bv;
-                float tv = s2.z * bw + s3.x * bu + s3.z * bv;
-                kd = tex_sample(sc, tex, tu, tv);
-            }
-            mask = mask * (kd * cw) / 0.5f;
-
-            // Light choice (renderer.cu:218-221) and estimateDirectLighting (:414-445):
-            // visible iff the closest hit of the shadow ray is the chosen light, tested as
-            // "the light is hit at t_L and nothing is hit before it (ties to lower id)".
-            float pick = u01(ra.w) * (float(sc.num_lights) - .00001f);
-            const uint32_t li = sc.lights[uint32_t(truncf(pick))];
-            V3 direct = v3(0.f, 0.f, 0.f);
-            if (h.rec != li) {
-                const float4* lp = sc.tris + size_t(li) * 3;
-                const float4 la = lp[0], lb = lp[1], lc = lp[2];
-                float lu = u01(rb.x), lv = u01(rb.y);
-                if (lu + lv >= 1.0f) {
-                    lu = 1.0f - lu;
-                    lv = 1.0f - lv;
-                }
-                V3 pt = v3(la) + v3(lb) * lu + v3(lc) * lv;
-                V3 so = position + normal * kEps;
-                V3 sd = normalize(pt - position);
-                if (COUNT) cnt.shadow++;
-                float tL, uL, vL;
-                if (COUNT) cnt.tris++;
-                if (mt_intersect(la, lb, lc, so, sd, tL, uL, vL)) {
-                    HitRec occ;
-                    occ.t = tL;
-                    occ.id = __float_as_uint(la.w);
-                    occ.rec = li;
-                    occ.u = uL;
-                    occ.v = vL;
-                    if (!traverse<true, COUNT>(sc, so, sd, kEps, occ, st, cnt)) {
-                        const float4* lsp = sc.shade + size_t(li) * 4;
-                        const float4 l0 = lsp[0], l1 = lsp[1], l2 = lsp[2], l3 = lsp[3];
-                        if (COUNT) cnt.hits++;
-                        float sa = l3.w;
-                        float d2 = tL * tL;
-                        float cosT = fabsf(dot(shading_normal(l0, l1, l2, uL, vL), sd * -1.0f));
-                        float wf = sa / (d2 * sc.lights_area) * cosT;
-                        const float4 lka = sc.mats[size_t(__float_as_uint(lb.w)) * 4 + 1];
-                        direct = direct + v3(lka) * wf;
-                    }
-                }
-            }
-            color = color + mask * direct;
-        }
-        if (bsdf == 1 /*SPECULAR*/) {
-            // sampleSpecularBSDF / reflect (renderer.cu:404-412)
-            inputDirection = outputDirection - normal * dot(outputDirection, normal) * 2.f;
-            mask = mask * v3(mks) / 1.0f;
-        }
-        previousBsdf = bsdf;
-        ro = position + inputDirection * kEps;
-        rd = inputDirection;
-        if (b >= pa.rr_start) {
-            float p = max_component(mask);
-            if (u01(rb.z) > p) break;
-            mask = mask / p;
-        }
+        if (!more) break;
     }
-    return color;
+    return ps.color;
 }
 
 template <bool COUNT>

@@ -1,21 +1,756 @@
-// Persistent wavefront path tracer and batched rendering (placeholder until the
-// queue kernels land): every entry point reports PRT_ERR_UNSUPPORTED.
+// Persistent wavefront path tracer with batching over views and scenes.
+//
+// Work is cut into chunks of at most `queue_slots` pixel-samples. A chunk is a
+// list of tiles (job, pixel range, sample range); slots are numbered tile by tile.
+// One chunk runs as a captured HIP graph:
+//
+//   setup -> generate -> [extend -> shade -> shadow] x max_depth -> resolve -> advance
+//
+//   extend   closest hit of every ray in the current queue (traversal only, low
+//            register pressure, persistent waves fetching 64 rays at a time)
+//   shade    shade_vertex() of every hit; survivors are compacted into the next
+//            queue and shadow rays into the shadow queue (wave ballot + one atomic
+//            per wave), finished paths write their colour to their slot
+//   shadow   occlusion test of every shadow ray; adds mask * direct
+//   resolve  adds the slots of each pixel to its running sum in sample order
+//
+// All paths of a chunk are at bounce `it` during iteration `it`, and the vertex
+// arithmetic is the megakernel's (path.hpp), so every image is bit-identical to
+// the megakernel's for the same seed, view and depth. The graph is replayed once
+// per chunk; kernels find their chunk through a device-side index, and every
+// counter is cleared by a kernel (no memset nodes in the graph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
 #include "../host/common.hpp"
+#include "path.hpp"
 #include "renderer.hpp"
 
 namespace prt {
 
-int render_wavefront(prt_renderer*, uint32_t) { return fail(PRT_ERR_UNSUPPORTED, "wavefront integrator not built"); }
-void release_wavefront(prt_renderer*) {}
+constexpr int kMaxIter = 64;
+enum WfCounter : uint32_t {
+    kExt0 = 0,      // ray count of queue 0
+    kExt1 = 1,      // ray count of queue 1
+    kShadow = 2,    // shadow ray count
+    kChunk = 3,     // index of the chunk being rendered
+    kPaths = 4,
+    kFetch = 8,     // per (kernel, iteration) fetch counters: 3 * kMaxIter
+    kStats = kFetch + 3 * kMaxIter,  // kCntNum 64-bit counters (two words each)
+    kNumCounters = kStats + 2 * kCntNum,
+};
+
+struct WfJob {
+    SceneArgs sc;
+    CamArgs cam;
+    PathArgs path;
+    uint32_t width, height;
+    float4* accum;
+};
+
+struct WfTile {
+    uint32_t job, p0, p1, s0, s1;
+    uint32_t slot0;  // first slot of the tile in its chunk
+    uint32_t pix0;   // first resolve thread of the tile in its chunk
+    uint32_t pad;
+};
+
+struct WfChunk {
+    uint32_t tile0, ntiles, paths, pixels;
+};
+
+struct WfArgs {
+    const WfJob* jobs;
+    const WfTile* tiles;
+    const WfChunk* chunks;
+    uint32_t* ctr;
+    float4* qo[2];   // o.xyz, slot
+    float4* qd[2];   // d.xyz, 0
+    float4* hits;    // t, u, v, record (0xFFFFFFFF = miss)
+    float4* pmask;   // mask.xyz, previous bsdf
+    float4* pcolor;  // color.xyz
+    uint4* pinfo;    // job, pixel, sample
+    float4* so;      // shadow origin.xyz, slot
+    float4* sd;      // shadow dir.xyz, tL
+    float4* sm;      // mask.xyz, light id
+    float4* sdir;    // direct.xyz, final flag
+    float4* result;  // per slot final colour
+    int* spill;
+    uint32_t spill_stride;
+    uint32_t max_depth;
+};
+
+struct WavefrontState {
+    int device = -1;
+    uint32_t capacity = 0;       // slots
+    uint32_t persistent = 0;     // threads of the persistent kernels (spill stride)
+    void* mem = nullptr;
+    size_t bytes = 0;
+    int* spill = nullptr;
+    uint32_t* ctr = nullptr;
+    WfArgs args{};
+    // plan buffers
+    WfJob* d_jobs = nullptr;
+    WfTile* d_tiles = nullptr;
+    WfChunk* d_chunks = nullptr;
+    size_t cap_jobs = 0, cap_tiles = 0, cap_chunks = 0;
+};
+
+namespace dev {
+
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* fetch) {
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(fetch, 64u);
+    return __shfl(base, 0, 64);
+}
+
+// Appends the lanes with `pred` to a queue: returns this lane's position.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* count) {
+    const unsigned long long m = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t n = uint32_t(__popcll(m));
+    uint32_t base = 0;
+    if (lane == 0 && n) base = atomicAdd(count, n);
+    base = __shfl(base, 0, 64);
+    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_stats(uint32_t* ctr, const Counts& c) {
+    unsigned long long v[6] = {c.primary, c.extension, c.shadow, c.nodes, c.tris, c.hits};
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(ctr + kStats);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned long long x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(out + k, x);
+    }
+}
+
+__device__ __forceinline__ uint32_t find_tile(const WfTile* t, uint32_t first, uint32_t n, uint32_t key, bool by_pix) {
+    uint32_t lo = first, hi = first + n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) / 2;
+        uint32_t v = by_pix ? t[mid].pix0 : t[mid].slot0;
+        if (v <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void wf_setup(WfArgs a) {
+    if (threadIdx.x != 0) return;
+    const uint32_t c = a.ctr[kChunk];
+    for (uint32_t k = 0; k < 3 * kMaxIter; ++k) a.ctr[kFetch + k] = 0;
+    a.ctr[kExt0] = a.chunks[c].paths;
+    a.ctr[kExt1] = 0;
+    a.ctr[kShadow] = 0;
+    a.ctr[kPaths] = a.chunks[c].paths;
+}
+
+__global__ void wf_advance(WfArgs a) {
+    if (threadIdx.x == 0) a.ctr[kChunk] = a.ctr[kChunk] + 1;
+}
+
+__global__ __launch_bounds__(256) void wf_generate(WfArgs a) {
+    const WfChunk ch = a.chunks[a.ctr[kChunk]];
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < ch.paths; slot += gridDim.x * blockDim.x) {
+        const WfTile t = a.tiles[find_tile(a.tiles, ch.tile0, ch.ntiles, slot, false)];
+        const uint32_t w = t.p1 - t.p0;
+        const uint32_t r = slot - t.slot0;
+        const uint32_t pixel = t.p0 + r % w, sample = t.s0 + r / w;
+        const WfJob& j = a.jobs[t.job];
+        const int x = int(pixel % j.width), y = int(pixel / j.width);
+        Rng rng{j.path.seed, j.path.view, pixel, sample};
+        V3 ro, rd;
+        camera_ray(j.cam, x, y, rng.block(0), ro, rd);
+        a.qo[0][slot] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(slot));
+        a.qd[0][slot] = make_float4(rd.x, rd.y, rd.z, 0.f);
+        a.pmask[slot] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
+        a.pcolor[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.pinfo[slot] = make_uint4(t.job, pixel, sample, 0u);
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
+    __shared__ int s_stack[kStackLdsQueue * kBlock];
+    const uint32_t q = it & 1u;
+    const uint32_t n = a.ctr[kExt0 + q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctr[kExt0 + (q ^ 1u)] = 0;  // the queue shade(it) fills
+        a.ctr[kShadow] = 0;
+    }
+    const uint32_t lane = threadIdx.x;
+    StackT<kStackLdsQueue> st;
+    st.lds = s_stack + lane;
+    st.spill_stride = a.spill_stride;
+    st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
+    st.sp = 0;
+    Counts cnt = {0, 0, 0, 0, 0, 0};
+    while (true) {
+        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it]);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+        const float4 o = a.qo[q][i], d = a.qd[q][i];
+        const uint32_t slot = __float_as_uint(o.w);
+        const WfJob& j = a.jobs[a.pinfo[slot].x];
+        if (COUNT) {
+            if (it == 0) cnt.primary++; else cnt.extension++;
+        }
+        HitRec h;
+        h.t = kFltMax;
+        h.id = 0;
+        h.rec = 0xFFFFFFFFu;
+        h.u = h.v = 0.f;
+        bool hit = traverse<false, COUNT>(j.sc, v3(o), v3(d), kEps, h, st, cnt);
+        a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
+    }
+    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
+    const uint32_t q = it & 1u;
+    const uint32_t n = a.ctr[kExt0 + q];
+    const uint32_t lane = threadIdx.x;
+    Counts cnt = {0, 0, 0, 0, 0, 0};
+    while (true) {
+        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it + 1]);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        bool more = false, want_shadow = false;
+        uint32_t slot = 0;
+        PathState ps;
+        ShadowReq sr;
+        if (valid) {
+            const float4 o = a.qo[q][i], d = a.qd[q][i], hh = a.hits[i];
+            slot = __float_as_uint(o.w);
+            const uint4 info = a.pinfo[slot];
+            const WfJob& j = a.jobs[info.x];
+            const float4 mk = a.pmask[slot], cl = a.pcolor[slot];
+            ps.ro = v3(o);
+            ps.rd = v3(d);
+            ps.mask = v3(mk);
+            ps.color = v3(cl);
+            ps.prev = __float_as_int(mk.w);
+            ps.b = it;
+            ps.input = it == 0 ? v3(0.f, 0.f, 0.f) : ps.rd;  // at b >= 1 the stale input is the ray direction
+            const uint32_t rec = __float_as_uint(hh.w);
+            if (rec == 0xFFFFFFFFu) {
+                path_miss(ps);
+            } else {
+                HitRec h;
+                h.t = hh.x;
+                h.u = hh.y;
+                h.v = hh.z;
+                h.rec = rec;
+                h.id = 0;
+                Rng rng{j.path.seed, j.path.view, info.y, info.z};
+                more = shade_vertex<COUNT>(j.sc, j.path, rng, ps, h, &want_shadow, sr, cnt);
+            }
+            a.pmask[slot] = make_float4(ps.mask.x, ps.mask.y, ps.mask.z, __int_as_float(ps.prev));
+            a.pcolor[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
+            if (!more && !want_shadow) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
+        }
+        const uint32_t qi = wave_append(more, &a.ctr[kExt0 + (q ^ 1u)]);
+        if (more) {
+            a.qo[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, __uint_as_float(slot));
+            a.qd[q ^ 1u][qi] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, 0.f);
+        }
+        const uint32_t si = wave_append(want_shadow, &a.ctr[kShadow]);
+        if (want_shadow) {
+            a.so[si] = make_float4(sr.so.x, sr.so.y, sr.so.z, __uint_as_float(slot));
+            a.sd[si] = make_float4(sr.sd.x, sr.sd.y, sr.sd.z, sr.tL);
+            a.sm[si] = make_float4(sr.mask.x, sr.mask.y, sr.mask.z, __uint_as_float(sr.lid));
+            a.sdir[si] = make_float4(sr.direct.x, sr.direct.y, sr.direct.z, __uint_as_float(more ? 0u : 1u));
+        }
+    }
+    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
+    __shared__ int s_stack[kStackLdsQueue * kBlock];
+    const uint32_t n = a.ctr[kShadow];
+    const uint32_t lane = threadIdx.x;
+    StackT<kStackLdsQueue> st;
+    st.lds = s_stack + lane;
+    st.spill_stride = a.spill_stride;
+    st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
+    st.sp = 0;
+    Counts cnt = {0, 0, 0, 0, 0, 0};
+    while (true) {
+        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it + 2]);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+        const float4 o = a.so[i], d = a.sd[i], m = a.sm[i], dr = a.sdir[i];
+        const uint32_t slot = __float_as_uint(o.w);
+        const WfJob& j = a.jobs[a.pinfo[slot].x];
+        HitRec occ;
+        occ.t = d.w;
+        occ.id = __float_as_uint(m.w);
+        occ.rec = 0;
+        occ.u = occ.v = 0.f;
+        const bool occluded = traverse<true, COUNT>(j.sc, v3(o), v3(d), kEps, occ, st, cnt);
+        ShadowReq sr;
+        sr.mask = v3(m);
+        sr.direct = v3(dr);
+        const float4 cl = a.pcolor[slot];
+        const V3 c = add_direct(v3(cl), sr, !occluded);
+        a.pcolor[slot] = make_float4(c.x, c.y, c.z, 0.f);
+        if (__float_as_uint(dr.w)) a.result[slot] = make_float4(c.x, c.y, c.z, 0.f);
+    }
+    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+}
+
+__global__ __launch_bounds__(256) void wf_resolve(WfArgs a) {
+    const WfChunk ch = a.chunks[a.ctr[kChunk]];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ch.pixels; k += gridDim.x * blockDim.x) {
+        const WfTile t = a.tiles[find_tile(a.tiles, ch.tile0, ch.ntiles, k, true)];
+        const uint32_t w = t.p1 - t.p0;
+        const uint32_t p = t.p0 + (k - t.pix0);
+        float4* acc = a.jobs[t.job].accum + p;
+        float4 s = *acc;
+        V3 sum = v3(s.x, s.y, s.z);
+        for (uint32_t smp = t.s0; smp < t.s1; ++smp) {
+            const float4 r = a.result[t.slot0 + (smp - t.s0) * w + (p - t.p0)];
+            sum = sum + v3(r);
+        }
+        *acc = make_float4(sum.x, sum.y, sum.z, 0.f);
+    }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+#define WF_TRY(expr)                                        \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);   \
+    } while (0)
+
+struct PlanJob {
+    uint32_t pixels;
+    uint32_t s_begin, s_end;
+};
+
+// Tiles of at most `cap` slots, packed in order into chunks of at most `cap` slots.
+void make_plan(const std::vector<PlanJob>& jobs, uint32_t cap, std::vector<WfTile>& tiles,
+               std::vector<WfChunk>& chunks) {
+    tiles.clear();
+    chunks.clear();
+    std::vector<WfTile> raw;
+    for (uint32_t j = 0; j < jobs.size(); ++j) {
+        const PlanJob& pj = jobs[j];
+        if (pj.s_end <= pj.s_begin || pj.pixels == 0) continue;
+        if (pj.pixels >= cap) {
+            for (uint32_t s = pj.s_begin; s < pj.s_end; ++s)
+                for (uint32_t p0 = 0; p0 < pj.pixels; p0 += cap)
+                    raw.push_back(WfTile{j, p0, std::min(pj.pixels, p0 + cap), s, s + 1, 0, 0, 0});
+        } else {
+            const uint32_t k = cap / pj.pixels;
+            for (uint32_t s = pj.s_begin; s < pj.s_end; s += k)
+                raw.push_back(WfTile{j, 0, pj.pixels, s, std::min(pj.s_end, s + k), 0, 0, 0});
+        }
+    }
+    WfChunk cur{0, 0, 0, 0};
+    for (WfTile t : raw) {
+        const uint32_t n = (t.p1 - t.p0) * (t.s1 - t.s0);
+        // A pixel may appear in only one tile of a chunk: resolve adds each tile's
+        // samples with one thread per pixel.
+        bool overlap = false;
+        for (uint32_t k = cur.tile0; k < tiles.size() && !overlap; ++k)
+            overlap = tiles[k].job == t.job && tiles[k].p0 < t.p1 && t.p0 < tiles[k].p1;
+        if (cur.ntiles && (cur.paths + n > cap || overlap)) {
+            chunks.push_back(cur);
+            cur = WfChunk{uint32_t(tiles.size()), 0, 0, 0};
+        }
+        t.slot0 = cur.paths;
+        t.pix0 = cur.pixels;
+        tiles.push_back(t);
+        cur.ntiles++;
+        cur.paths += n;
+        cur.pixels += t.p1 - t.p0;
+    }
+    if (cur.ntiles) chunks.push_back(cur);
+}
+
+int persistent_blocks(const void* kernel, uint32_t blocks_per_cu, int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ <= 0) occ = 8;
+    uint32_t per = blocks_per_cu ? blocks_per_cu : uint32_t(occ);
+    return int(std::max<uint32_t>(1, per * uint32_t(prop.multiProcessorCount)));
+}
+
+template <class T>
+int ensure_buffer(T*& p, size_t& cap, size_t n) {
+    if (cap >= n && p) return PRT_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size_t want = std::max<size_t>(n, 16);
+    WF_TRY(hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T)));
+    cap = want;
+    return PRT_OK;
+}
+
+int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t persistent) {
+    if (w.mem && w.capacity >= capacity && w.persistent >= persistent && w.device == device) return PRT_OK;
+    if (w.mem) (void)hipFree(w.mem);
+    if (w.spill) (void)hipFree(w.spill);
+    if (w.ctr) (void)hipFree(w.ctr);
+    w.mem = nullptr;
+    w.spill = nullptr;
+    w.ctr = nullptr;
+    // per slot: 2 queues x 32 B, hit 16, mask 16, colour 16, info 16, shadow 64, result 16
+    const size_t per = 2 * 32 + 16 + 16 + 16 + 16 + 64 + 16;
+    w.bytes = per * size_t(capacity);
+    WF_TRY(hipMalloc(&w.mem, w.bytes));
+    WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.ctr), kNumCounters * sizeof(uint32_t)));
+    WF_TRY(hipMemset(w.ctr, 0, kNumCounters * sizeof(uint32_t)));
+    const size_t spill_entries = size_t(kStackMax - kStackLdsQueue) * persistent;
+    WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.spill), spill_entries * sizeof(int)));
+    char* p = static_cast<char*>(w.mem);
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += bytes;
+        return r;
+    };
+    const size_t C = capacity;
+    WfArgs& a = w.args;
+    a.qo[0] = static_cast<float4*>(take(16 * C));
+    a.qd[0] = static_cast<float4*>(take(16 * C));
+    a.qo[1] = static_cast<float4*>(take(16 * C));
+    a.qd[1] = static_cast<float4*>(take(16 * C));
+    a.hits = static_cast<float4*>(take(16 * C));
+    a.pmask = static_cast<float4*>(take(16 * C));
+    a.pcolor = static_cast<float4*>(take(16 * C));
+    a.pinfo = static_cast<uint4*>(take(16 * C));
+    a.so = static_cast<float4*>(take(16 * C));
+    a.sd = static_cast<float4*>(take(16 * C));
+    a.sm = static_cast<float4*>(take(16 * C));
+    a.sdir = static_cast<float4*>(take(16 * C));
+    a.result = static_cast<float4*>(take(16 * C));
+    a.ctr = w.ctr;
+    a.spill = w.spill;
+    a.spill_stride = persistent;
+    w.capacity = capacity;
+    w.persistent = persistent;
+    w.device = device;
+    return PRT_OK;
+}
+
+void free_state(WavefrontState& w) {
+    void* ptrs[] = {w.mem, w.spill, w.ctr, w.d_jobs, w.d_tiles, w.d_chunks};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    w = WavefrontState();
+}
+
+struct RunStats {
+    unsigned long long c[kCntNum];
+    double ms;
+};
+
+// Renders every job of the plan on `stream`. Returns PRT_OK or an error.
+int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::vector<WfJob>& jobs,
+                  const std::vector<PlanJob>& pjobs, uint32_t max_depth, uint32_t capacity, uint32_t blocks_per_cu,
+                  bool count, RunStats* rs) {
+    if (max_depth == 0 || max_depth > uint32_t(kMaxIter)) return fail(PRT_ERR_INVALID_ARG, "max_depth out of range");
+    std::vector<WfTile> tiles;
+    std::vector<WfChunk> chunks;
+    make_plan(pjobs, capacity, tiles, chunks);
+    if (chunks.empty()) return PRT_OK;
+    uint32_t need = 0;
+    for (auto& c : chunks) need = std::max(need, c.paths);
+    const void* kext = count ? reinterpret_cast<const void*>(dev::wf_extend<true>)
+                             : reinterpret_cast<const void*>(dev::wf_extend<false>);
+    const int pblocks = persistent_blocks(kext, blocks_per_cu, device);
+    int rc = ensure_state(w, device, need, uint32_t(pblocks) * kBlock);
+    if (rc != PRT_OK) return rc;
+    rc = ensure_buffer(w.d_jobs, w.cap_jobs, jobs.size());
+    if (rc == PRT_OK) rc = ensure_buffer(w.d_tiles, w.cap_tiles, tiles.size());
+    if (rc == PRT_OK) rc = ensure_buffer(w.d_chunks, w.cap_chunks, chunks.size());
+    if (rc != PRT_OK) return rc;
+    WF_TRY(hipMemcpyAsync(w.d_jobs, jobs.data(), jobs.size() * sizeof(WfJob), hipMemcpyHostToDevice, stream));
+    WF_TRY(hipMemcpyAsync(w.d_tiles, tiles.data(), tiles.size() * sizeof(WfTile), hipMemcpyHostToDevice, stream));
+    WF_TRY(hipMemcpyAsync(w.d_chunks, chunks.data(), chunks.size() * sizeof(WfChunk), hipMemcpyHostToDevice, stream));
+    WF_TRY(hipMemsetAsync(w.ctr, 0, kNumCounters * sizeof(uint32_t), stream));
+    WfArgs a = w.args;
+    a.jobs = w.d_jobs;
+    a.tiles = w.d_tiles;
+    a.chunks = w.d_chunks;
+    a.max_depth = max_depth;
+    a.spill_stride = uint32_t(pblocks) * kBlock;
+
+    hipDeviceProp_t prop;
+    WF_TRY(hipGetDeviceProperties(&prop, device));
+    const int gblocks = prop.multiProcessorCount * 8;
+
+    // Capture one chunk's launch sequence.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    WF_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(dev::wf_setup, dim3(1), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(dev::wf_generate, dim3(gblocks), dim3(256), 0, stream, a);
+    for (uint32_t it = 0; it < max_depth; ++it) {
+        if (count) {
+            hipLaunchKernelGGL(dev::wf_extend<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shade<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shadow<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        } else {
+            hipLaunchKernelGGL(dev::wf_extend<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shade<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shadow<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        }
+    }
+    hipLaunchKernelGGL(dev::wf_resolve, dim3(gblocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(dev::wf_advance, dim3(1), dim3(64), 0, stream, a);
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, stream);
+    for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) e = hipGraphLaunch(exec, stream);
+    (void)hipEventRecord(e1, stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "wavefront launch");
+    if (rs) {
+        rs->ms = ms;
+        WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
+    }
+    return PRT_OK;
+}
+
+uint32_t default_capacity(uint64_t total_paths) {
+    const uint64_t cap = 1ull << 23;  // 8M slots (about 1.6 GB of queues and state)
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(cap, total_paths)));
+}
+
+}  // namespace
+
+int render_wavefront(prt_renderer* r, uint32_t n) {
+    if (!r->wf) r->wf = new WavefrontState();
+    WfJob j;
+    j.sc = r->sc;
+    j.cam = cam_args(r->cam);
+    j.path = PathArgs{r->cfg.seed, r->cfg.view_id, r->cfg.max_depth, r->cfg.rr_start};
+    j.width = r->w;
+    j.height = r->h;
+    j.accum = r->accum;
+    std::vector<WfJob> jobs = {j};
+    std::vector<PlanJob> pj = {PlanJob{r->w * r->h, r->samples_done, r->samples_done + n}};
+    const uint64_t total = uint64_t(r->w) * r->h * n;
+    const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
+    RunStats rs;
+    int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap, 0,
+                           r->cfg.count_rays != 0, &rs);
+    if (rc != PRT_OK) return rc;
+    // Fold the wavefront's counters into the renderer's.
+    unsigned long long host[kCntNum];
+    WF_TRY(hipMemcpy(host, r->counters, sizeof host, hipMemcpyDeviceToHost));
+    for (int k = 0; k < kCntNum; ++k) host[k] += rs.c[k];
+    WF_TRY(hipMemcpy(r->counters, host, sizeof host, hipMemcpyHostToDevice));
+    r->last_ms = rs.ms;
+    r->samples_done += n;
+    r->samples_total += total;
+    return PRT_OK;
+}
+
+void release_wavefront(prt_renderer* r) {
+    if (!r->wf) return;
+    free_state(*r->wf);
+    delete r->wf;
+    r->wf = nullptr;
+}
+
+namespace {
+
+struct BatchCtx {
+    std::vector<WfJob> jobs;
+    std::vector<PlanJob> pjobs;
+    std::vector<float4*> accum;
+    uint32_t max_depth = 0;
+    uint64_t total = 0;
+};
+
+int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCtx& b) {
+    if (!jobs || n == 0) return fail(PRT_ERR_INVALID_ARG, "no jobs");
+    b.max_depth = c.max_depth;
+    for (uint32_t i = 0; i < n; ++i) {
+        const prt_job& jb = jobs[i];
+        if (!jb.scene) return fail(PRT_ERR_INVALID_ARG, "job without a scene");
+        if (jb.scene->lights.empty()) return fail(PRT_ERR_NO_EMITTERS, "job scene has no emitters");
+        if (jb.camera.xpixels <= 0 || jb.camera.ypixels <= 0) return fail(PRT_ERR_INVALID_ARG, "job camera size");
+        WfJob w;
+        int rc = device_scene(jb.scene, c.device, w.sc);
+        if (rc != PRT_OK) return rc;
+        w.cam = cam_args(jb.camera);
+        w.path = PathArgs{c.seed, jb.view_id, c.max_depth, c.rr_start};
+        w.width = uint32_t(jb.camera.xpixels);
+        w.height = uint32_t(jb.camera.ypixels);
+        const size_t px = size_t(w.width) * w.height;
+        float4* acc = nullptr;
+        WF_TRY(hipMalloc(reinterpret_cast<void**>(&acc), px * sizeof(float4)));
+        b.accum.push_back(acc);
+        WF_TRY(hipMemset(acc, 0, px * sizeof(float4)));
+        w.accum = acc;
+        b.jobs.push_back(w);
+        b.pjobs.push_back(PlanJob{uint32_t(px), 0, jb.spp});
+        b.total += uint64_t(px) * jb.spp;
+    }
+    return PRT_OK;
+}
+
+void batch_release(BatchCtx& b) {
+    for (float4* p : b.accum)
+        if (p) (void)hipFree(p);
+    b.accum.clear();
+}
+
+}  // namespace
 
 }  // namespace prt
 
-extern "C" int prt_render_batch(prt_job*, uint32_t, const prt_render_config*, const prt_tune*, prt_stats*) {
-    return prt::fail(PRT_ERR_UNSUPPORTED, "batched wavefront not built");
+using namespace prt;
+
+extern "C" int prt_render_batch(prt_job* jobs, uint32_t n, const prt_render_config* cfg, const prt_tune* tune,
+                                prt_stats* stats) {
+    clear_error();
+    prt_render_config c;
+    if (cfg) c = *cfg;
+    else prt_render_config_default(&c);
+    int rc = set_device(c.device);
+    if (rc != PRT_OK) return rc;
+    PRT_TRY_BEGIN
+    BatchCtx b;
+    rc = batch_prepare(jobs, n, c, b);
+    if (rc != PRT_OK) {
+        batch_release(b);
+        return rc;
+    }
+    hipStream_t s;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        batch_release(b);
+        return hip_fail(e, "stream");
+    }
+    WavefrontState w;
+    const uint32_t cap = tune && tune->queue_slots ? tune->queue_slots : default_capacity(b.total);
+    RunStats rs;
+    rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, cap, tune ? tune->blocks_per_cu : 0,
+                       c.count_rays != 0, &rs);
+    if (rc == PRT_OK) {
+        for (uint32_t i = 0; i < n && rc == PRT_OK; ++i) {
+            if (!jobs[i].out_rgb) continue;
+            const size_t px = size_t(b.jobs[i].width) * b.jobs[i].height;
+            float* tmp = nullptr;
+            e = hipMalloc(reinterpret_cast<void**>(&tmp), px * 3 * sizeof(float));
+            if (e == hipSuccess) e = launch_mean(b.accum[i], uint32_t(px), jobs[i].spp, tmp, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = hipMemcpy(jobs[i].out_rgb, tmp, px * 3 * sizeof(float), hipMemcpyDeviceToHost);
+            if (tmp) (void)hipFree(tmp);
+            if (e != hipSuccess) rc = hip_fail(e, "batch readback");
+        }
+    }
+    if (rc == PRT_OK && stats) {
+        stats->primary_rays = rs.c[kCntPrimary];
+        stats->extension_rays = rs.c[kCntExtension];
+        stats->shadow_rays = rs.c[kCntShadow];
+        stats->nodes_visited = rs.c[kCntNodes];
+        stats->tris_tested = rs.c[kCntTris];
+        stats->hits = rs.c[kCntHits];
+        stats->samples = b.total;
+        stats->last_render_ms = rs.ms;
+    }
+    free_state(w);
+    (void)hipStreamDestroy(s);
+    batch_release(b);
+    return rc;
+    PRT_TRY_END
 }
 
-extern "C" int prt_autotune(prt_job*, uint32_t, const prt_render_config*, prt_tune*, double*) {
-    return prt::fail(PRT_ERR_UNSUPPORTED, "autotuner not built");
+extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* cfg, prt_tune* best,
+                            double* best_ms) {
+    clear_error();
+    if (!best) return fail(PRT_ERR_INVALID_ARG, "null argument");
+    prt_render_config c;
+    if (cfg) c = *cfg;
+    else prt_render_config_default(&c);
+    c.count_rays = 0;
+    int rc = set_device(c.device);
+    if (rc != PRT_OK) return rc;
+    PRT_TRY_BEGIN
+    // Trial workload: every job at min(spp, 2) samples per pixel.
+    std::vector<prt_job> trial(jobs, jobs + n);
+    uint64_t paths = 0;
+    for (auto& t : trial) {
+        t.spp = std::min<uint32_t>(t.spp, 2);
+        t.out_rgb = nullptr;
+        paths += uint64_t(t.camera.xpixels) * t.camera.ypixels * t.spp;
+    }
+    BatchCtx b;
+    rc = batch_prepare(trial.data(), n, c, b);
+    if (rc != PRT_OK) {
+        batch_release(b);
+        return rc;
+    }
+    hipStream_t s;
+    WF_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    WavefrontState w;
+    double bms = 1e30;
+    prt_tune bt{};
+    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    const uint32_t bpcs[] = {4, 8, 12, 16};
+    for (uint32_t cap : caps) {
+        if (cap > 2 * paths && cap != caps[0]) continue;
+        for (uint32_t bpc : bpcs) {
+            RunStats rs;
+            rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, cap, bpc, false, &rs);
+            if (rc != PRT_OK) break;
+            if (rs.ms < bms) {
+                bms = rs.ms;
+                bt = prt_tune{cap, 0, kBlock, bpc};
+            }
+        }
+        if (rc != PRT_OK) break;
+    }
+    free_state(w);
+    (void)hipStreamDestroy(s);
+    batch_release(b);
+    if (rc != PRT_OK) return rc;
+    *best = bt;
+    if (best_ms) *best_ms = bms;
+    return PRT_OK;
+    PRT_TRY_END
 }
 
 extern "C" int prt_adjoint(prt_renderer*, const float*, const float*, uint32_t, float*, float*, float*) {
