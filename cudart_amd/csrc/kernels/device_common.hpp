@@ -152,6 +152,7 @@ struct HitRec {
     float t, u, v;
     uint32_t rec;  // record index (BVH order)
     uint32_t id;   // original triangle id, 0 = none yet
+    uint32_t mat;  // material of the record
 };
 
 // Moller-Trumbore exactly as Triangle::intersect (src/geometry/geometry.cu:38-65).
@@ -234,6 +235,7 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
                     best.v = v;
                     best.rec = first + k;
                     best.id = id;
+                    best.mat = __float_as_uint(b.w);
                     found = true;
                 }
             }

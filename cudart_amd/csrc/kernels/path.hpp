@@ -1,11 +1,15 @@
 // One path vertex of samplePixel (src/renderer/renderer.cu:119-262), shared by
-// the megakernel and the wavefront shade kernel so both integrators perform the
-// same operations in the same order.
+// the megakernel and the wavefront kernels so both integrators perform the same
+// operations in the same order.
 //
-// The caller traces the ray (closest hit), adds the sky on a miss, calls
-// shade_vertex on a hit, then — when it returns a shadow request — tests the
-// shadow ray and adds mask * direct to the colour, with direct = 0 when the light
-// is occluded, exactly as estimateDirectLighting's caller does (renderer.cu:235).
+// Per vertex the caller
+//   1. traces the ray (closest hit); on a miss calls path_miss and stops;
+//   2. calls shade_vertex, which adds emission, samples the BSDF, updates the
+//      throughput, picks the light sample and decides Russian roulette;
+//   3. when shade_vertex asks for next-event estimation, calls nee_prepare (the
+//      light-sample geometry of estimateDirectLighting, renderer.cu:414-445) and,
+//      if the light sample is reachable, tests the shadow ray for occlusion; then
+//      adds mask * direct with direct = 0 when occluded (renderer.cu:235).
 #pragma once
 
 #include "device_common.hpp"
@@ -21,18 +25,20 @@ struct PathState {
     uint32_t b;    // bounce index
 };
 
+// Light sample chosen at a diffuse vertex.
+struct NeeSample {
+    V3 position, normal;
+    V3 mask;         // throughput after this vertex's diffuse weight
+    uint32_t li;     // light record
+    float lu, lv;    // point on the light (already folded when lu + lv >= 1)
+};
+
 struct ShadowReq {
     V3 so, sd;       // shadow ray
     float tL;        // distance to the light sample (the occlusion bound)
     uint32_t lid;    // original id of the light triangle (ties)
-    V3 mask;         // path throughput after this vertex's diffuse weight
     V3 direct;       // 0 + lightColor * wf, the value when the light is visible
 };
-
-// color + mask * direct (visible) or color + mask * 0 (occluded), renderer.cu:235.
-__device__ __forceinline__ V3 add_direct(V3 color, const ShadowReq& sr, bool visible) {
-    return color + sr.mask * (visible ? sr.direct : v3(0.f, 0.f, 0.f));
-}
 
 __device__ __forceinline__ void path_init(PathState& ps, V3 ro, V3 rd) {
     ps.ro = ro;
@@ -48,30 +54,44 @@ __device__ __forceinline__ void path_miss(PathState& ps) {
     ps.color = ps.color + ps.mask * v3(0.0f, 0.025f, 0.05f);  // skybox, renderer.cu:64-70,147-149
 }
 
+// color + mask * direct (visible) or color + mask * 0 (occluded or unreachable).
+__device__ __forceinline__ V3 add_direct(V3 color, V3 mask, V3 direct, bool visible) {
+    return color + mask * (visible ? direct : v3(0.f, 0.f, 0.f));
+}
+
 // Shades the hit `h` of the current ray. Returns true when the path continues
-// (next ray in ps.ro / ps.rd, bounce incremented); sets *want_shadow and fills
-// `sr` when a shadow ray must be tested.
-template <bool COUNT>
+// (next ray in ps.ro / ps.rd, bounce incremented); sets *want_nee and fills `ns`
+// at a diffuse vertex whose chosen light is another triangle.
+// EAGER evaluates both Philox blocks of the bounce up front, independent of any
+// load (latency hiding in the shade kernel); otherwise only the blocks the
+// material needs are evaluated (fewer registers in the megakernel). The values
+// are the same either way: a block depends only on (pixel, sample, bounce).
+template <bool COUNT, bool EAGER>
 __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs& pa, const Rng& rng, PathState& ps,
-                                             const HitRec& h, bool* want_shadow, ShadowReq& sr, Counts& cnt) {
-    *want_shadow = false;
+                                             const HitRec& h, bool* want_nee, NeeSample& ns, Counts& cnt) {
+    *want_nee = false;
     const uint32_t b = ps.b;
+    uint4 ra = make_uint4(0, 0, 0, 0), rb = make_uint4(0, 0, 0, 0);
+    if (EAGER) {
+        ra = rng.block(1u + 2u * b);
+        rb = rng.block(2u + 2u * b);
+    }
     if (COUNT) cnt.hits++;
-    const float4* tp = sc.tris + size_t(h.rec) * 3;
-    const uint32_t m = __float_as_uint(tp[1].w);
-    const float4* mp = sc.mats + size_t(m) * 4;
+    const float4* mp = sc.mats + size_t(h.mat) * 4;
+    const float4* shp = sc.shade + size_t(h.rec) * 4;
     const float4 mkd = mp[0], mka = mp[1], mks = mp[2], maux = mp[3];
+    const float4 s0 = shp[0], s1 = shp[1], s2 = shp[2];
+    if (!EAGER) {
+        const int k = __float_as_int(mkd.w);
+        if (k == 0 || k == 2 || k == 4) ra = rng.block(1u + 2u * b);
+    }
     if (b == 0 || ps.prev == 1 /*SPECULAR*/ || ps.prev == 4 /*REFRACTIVE*/)
         ps.color = ps.color + ps.mask * v3(mka) * sc.inv_lights_area;
 
-    const float4* shp = sc.shade + size_t(h.rec) * 4;
-    const float4 s0 = shp[0], s1 = shp[1], s2 = shp[2];
     const V3 normal = shading_normal(s0, s1, s2, h.u, h.v);
     const V3 position = ps.ro + ps.rd * h.t + normal * kEps2;
     const V3 outputDirection = normalize(ps.rd);
     int bsdf = __float_as_int(mkd.w);
-    uint4 ra = make_uint4(0, 0, 0, 0);
-    if (bsdf == 0 || bsdf == 2 || bsdf == 4) ra = rng.block(1u + 2u * b);
 
     if (bsdf == 2 /*DIFFSPEC*/) {
         float p = maux.y;
@@ -125,8 +145,7 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
             }
         }
     }
-    uint4 rb = make_uint4(0, 0, 0, 0);
-    if (bsdf == 0 || b >= pa.rr_start) rb = rng.block(2u + 2u * b);
+    if (!EAGER && (bsdf == 0 || b >= pa.rr_start)) rb = rng.block(2u + 2u * b);
     if (bsdf == 0 /*DIFFUSE*/) {
         // sampleDiffuseBSDF (renderer.cu:375-402): pdf 0.5, weight kd * cos (Q1).
         float r1 = float(2 * M_PI * double(u01(ra.y)));
@@ -152,48 +171,26 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
         }
         ps.mask = ps.mask * (kd * cw) / 0.5f;
 
-        // Light choice (renderer.cu:218-221) and estimateDirectLighting (:414-445).
-        // Visible iff the closest hit of the shadow ray is the chosen light: the light
-        // is hit at tL and nothing has (t, id) lexicographically below (tL, light id).
+        // Light choice (renderer.cu:218-221); estimateDirectLighting returns 0 when the
+        // shading triangle is the chosen light (:421-423).
         float pick = u01(ra.w) * (float(sc.num_lights) - .00001f);
         const uint32_t li = sc.lights[uint32_t(truncf(pick))];
-        bool shadow = false;
         if (h.rec != li) {
-            const float4* lp = sc.tris + size_t(li) * 3;
-            const float4 la = lp[0], lb = lp[1], lc = lp[2];
             float lu = u01(rb.x), lv = u01(rb.y);
             if (lu + lv >= 1.0f) {
                 lu = 1.0f - lu;
                 lv = 1.0f - lv;
             }
-            V3 pt = v3(la) + v3(lb) * lu + v3(lc) * lv;
-            V3 so = position + normal * kEps;
-            V3 sd = normalize(pt - position);
-            if (COUNT) {
-                cnt.shadow++;
-                cnt.tris++;
-            }
-            float tL, uL, vL;
-            if (mt_intersect(la, lb, lc, so, sd, tL, uL, vL)) {
-                const float4* lsp = sc.shade + size_t(li) * 4;
-                const float4 l0 = lsp[0], l1 = lsp[1], l2 = lsp[2], l3 = lsp[3];
-                if (COUNT) cnt.hits++;
-                float sa = l3.w;
-                float d2 = tL * tL;
-                float cosT = fabsf(dot(shading_normal(l0, l1, l2, uL, vL), sd * -1.0f));
-                float wf = sa / (d2 * sc.lights_area) * cosT;
-                const float4 lka = sc.mats[size_t(__float_as_uint(lb.w)) * 4 + 1];
-                sr.so = so;
-                sr.sd = sd;
-                sr.tL = tL;
-                sr.lid = __float_as_uint(la.w);
-                sr.mask = ps.mask;
-                sr.direct = v3(0.f, 0.f, 0.f) + v3(lka) * wf;
-                shadow = true;
-            }
+            ns.position = position;
+            ns.normal = normal;
+            ns.mask = ps.mask;
+            ns.li = li;
+            ns.lu = lu;
+            ns.lv = lv;
+            *want_nee = true;
+        } else {
+            ps.color = ps.color + ps.mask * v3(0.f, 0.f, 0.f);
         }
-        if (shadow) *want_shadow = true;  // the caller adds mask * direct after the visibility test
-        else ps.color = ps.color + ps.mask * v3(0.f, 0.f, 0.f);  // light missed or is this triangle
     }
     if (bsdf == 1 /*SPECULAR*/) {
         // sampleSpecularBSDF / reflect (renderer.cu:404-412)
@@ -210,6 +207,37 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
     }
     ps.b = b + 1;
     return ps.b < pa.max_depth;
+}
+
+// Shadow ray toward the light sample (estimateDirectLighting, renderer.cu:414-445).
+// Returns false when the ray misses the light triangle itself: its closest hit
+// cannot be the light, so the sample contributes mask * 0. Otherwise the light is
+// visible iff nothing has (t, id) below (tL, light id) along the ray.
+template <bool COUNT>
+__device__ __forceinline__ bool nee_prepare(const SceneArgs& sc, const NeeSample& ns, ShadowReq& sr, Counts& cnt) {
+    const float4* lp = sc.tris + size_t(ns.li) * 3;
+    const float4 la = lp[0], lb = lp[1], lc = lp[2];
+    V3 pt = v3(la) + v3(lb) * ns.lu + v3(lc) * ns.lv;
+    sr.so = ns.position + ns.normal * kEps;
+    sr.sd = normalize(pt - ns.position);
+    if (COUNT) {
+        cnt.shadow++;
+        cnt.tris++;
+    }
+    float tL, uL, vL;
+    if (!mt_intersect(la, lb, lc, sr.so, sr.sd, tL, uL, vL)) return false;
+    const float4* lsp = sc.shade + size_t(ns.li) * 4;
+    const float4 l0 = lsp[0], l1 = lsp[1], l2 = lsp[2], l3 = lsp[3];
+    if (COUNT) cnt.hits++;
+    float sa = l3.w;
+    float d2 = tL * tL;
+    float cosT = fabsf(dot(shading_normal(l0, l1, l2, uL, vL), sr.sd * -1.0f));
+    float wf = sa / (d2 * sc.lights_area) * cosT;
+    const float4 lka = sc.mats[size_t(__float_as_uint(lb.w)) * 4 + 1];
+    sr.tL = tL;
+    sr.lid = __float_as_uint(la.w);
+    sr.direct = v3(0.f, 0.f, 0.f) + v3(lka) * wf;
+    return true;
 }
 
 }  // namespace dev

@@ -27,22 +27,29 @@ __device__ V3 sample_path(const SceneArgs& sc, const CamArgs& cam, const PathArg
         h.t = kFltMax;
         h.id = 0;
         h.rec = 0;
+        h.mat = 0;
         h.u = h.v = 0.f;
         if (!traverse<false, COUNT>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) {
             path_miss(ps);
             break;
         }
-        bool want_shadow;
-        ShadowReq sr;
-        const bool more = shade_vertex<COUNT>(sc, pa, rng, ps, h, &want_shadow, sr, cnt);
-        if (want_shadow) {
-            HitRec occ;
-            occ.t = sr.tL;
-            occ.id = sr.lid;
-            occ.rec = 0;
-            occ.u = occ.v = 0.f;
-            const bool occluded = traverse<true, COUNT>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
-            ps.color = add_direct(ps.color, sr, !occluded);
+        bool want_nee;
+        NeeSample ns;
+        const bool more = shade_vertex<COUNT, false>(sc, pa, rng, ps, h, &want_nee, ns, cnt);
+        if (want_nee) {
+            ShadowReq sr;
+            sr.direct = v3(0.f, 0.f, 0.f);
+            bool visible = false;
+            if (nee_prepare<COUNT>(sc, ns, sr, cnt)) {
+                HitRec occ;
+                occ.t = sr.tL;
+                occ.id = sr.lid;
+                occ.rec = 0;
+                occ.mat = 0;
+                occ.u = occ.v = 0.f;
+                visible = !traverse<true, COUNT>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
+            }
+            ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
         }
         if (!more) break;
     }
@@ -63,7 +70,7 @@ __device__ __forceinline__ void flush_counts(unsigned long long* out, const Coun
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void megakernel(MegaArgs a) {
+__global__ __launch_bounds__(kBlock, 4) void megakernel(MegaArgs a) {
     __shared__ int s_stack[kStackLds * kBlock];
     const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
     if (tile >= a.tiles) return;
@@ -109,12 +116,13 @@ __global__ __launch_bounds__(kBlock) void first_hit_kernel(FirstHitArgs a) {
     h.t = kFltMax;
     h.id = 0;
     h.rec = 0;
+    h.mat = 0;
     h.u = h.v = 0.f;
     Counts cnt;
     bool hit = traverse<false, false>(a.sc, ro, rd, kEps, h, st, cnt);
     const size_t idx = size_t(y) * a.width + size_t(x);
     if (a.tri) a.tri[idx] = hit ? int32_t(h.id) : -1;
-    if (a.mat) a.mat[idx] = hit ? int32_t(__float_as_uint(a.sc.tris[size_t(h.rec) * 3 + 1].w)) : -1;
+    if (a.mat) a.mat[idx] = hit ? int32_t(h.mat) : -1;
     if (a.t) a.t[idx] = hit ? h.t : kFltMax;
 }
 

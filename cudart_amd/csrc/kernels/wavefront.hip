@@ -73,13 +73,14 @@ struct WfArgs {
     float4* qo[2];   // o.xyz, slot
     float4* qd[2];   // d.xyz, 0
     float4* hits;    // t, u, v, record (0xFFFFFFFF = miss)
+    uint32_t* hmat;  // material of the hit record
     float4* pmask;   // mask.xyz, previous bsdf
     float4* pcolor;  // color.xyz
     uint4* pinfo;    // job, pixel, sample
-    float4* so;      // shadow origin.xyz, slot
-    float4* sd;      // shadow dir.xyz, tL
-    float4* sm;      // mask.xyz, light id
-    float4* sdir;    // direct.xyz, final flag
+    float4* so;      // light sample: position.xyz, slot
+    float4* sd;      // normal.xyz, lu
+    float4* sm;      // mask.xyz, light record
+    float4* sdir;    // lv, final flag, 0, 0
     float4* result;  // per slot final colour
     int* spill;
     uint32_t spill_stride;
@@ -210,9 +211,11 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
         h.t = kFltMax;
         h.id = 0;
         h.rec = 0xFFFFFFFFu;
+        h.mat = 0;
         h.u = h.v = 0.f;
         bool hit = traverse<false, COUNT>(j.sc, v3(o), v3(d), kEps, h, st, cnt);
         a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
+        a.hmat[i] = h.mat;
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
@@ -223,15 +226,14 @@ __global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
     const uint32_t n = a.ctr[kExt0 + q];
     const uint32_t lane = threadIdx.x;
     Counts cnt = {0, 0, 0, 0, 0, 0};
-    while (true) {
-        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it + 1]);
-        if (base >= n) break;
+    // Static, wave-uniform partition of the queue (shading cost varies little).
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + lane;
         const bool valid = i < n;
-        bool more = false, want_shadow = false;
+        bool more = false, want_nee = false;
         uint32_t slot = 0;
         PathState ps;
-        ShadowReq sr;
+        NeeSample ns;
         if (valid) {
             const float4 o = a.qo[q][i], d = a.qd[q][i], hh = a.hits[i];
             slot = __float_as_uint(o.w);
@@ -255,24 +257,25 @@ __global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
                 h.v = hh.z;
                 h.rec = rec;
                 h.id = 0;
+                h.mat = a.hmat[i];
                 Rng rng{j.path.seed, j.path.view, info.y, info.z};
-                more = shade_vertex<COUNT>(j.sc, j.path, rng, ps, h, &want_shadow, sr, cnt);
+                more = shade_vertex<COUNT, true>(j.sc, j.path, rng, ps, h, &want_nee, ns, cnt);
             }
             a.pmask[slot] = make_float4(ps.mask.x, ps.mask.y, ps.mask.z, __int_as_float(ps.prev));
             a.pcolor[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
-            if (!more && !want_shadow) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
+            if (!more && !want_nee) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
         }
         const uint32_t qi = wave_append(more, &a.ctr[kExt0 + (q ^ 1u)]);
         if (more) {
             a.qo[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, __uint_as_float(slot));
             a.qd[q ^ 1u][qi] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, 0.f);
         }
-        const uint32_t si = wave_append(want_shadow, &a.ctr[kShadow]);
-        if (want_shadow) {
-            a.so[si] = make_float4(sr.so.x, sr.so.y, sr.so.z, __uint_as_float(slot));
-            a.sd[si] = make_float4(sr.sd.x, sr.sd.y, sr.sd.z, sr.tL);
-            a.sm[si] = make_float4(sr.mask.x, sr.mask.y, sr.mask.z, __uint_as_float(sr.lid));
-            a.sdir[si] = make_float4(sr.direct.x, sr.direct.y, sr.direct.z, __uint_as_float(more ? 0u : 1u));
+        const uint32_t si = wave_append(want_nee, &a.ctr[kShadow]);
+        if (want_nee) {
+            a.so[si] = make_float4(ns.position.x, ns.position.y, ns.position.z, __uint_as_float(slot));
+            a.sd[si] = make_float4(ns.normal.x, ns.normal.y, ns.normal.z, ns.lu);
+            a.sm[si] = make_float4(ns.mask.x, ns.mask.y, ns.mask.z, __uint_as_float(ns.li));
+            a.sdir[si] = make_float4(ns.lv, __uint_as_float(more ? 0u : 1u), 0.f, 0.f);
         }
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
@@ -294,22 +297,32 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
         if (base >= n) break;
         const uint32_t i = base + lane;
         if (i >= n) continue;
-        const float4 o = a.so[i], d = a.sd[i], m = a.sm[i], dr = a.sdir[i];
+        const float4 o = a.so[i], d = a.sd[i], m = a.sm[i], e = a.sdir[i];
         const uint32_t slot = __float_as_uint(o.w);
         const WfJob& j = a.jobs[a.pinfo[slot].x];
-        HitRec occ;
-        occ.t = d.w;
-        occ.id = __float_as_uint(m.w);
-        occ.rec = 0;
-        occ.u = occ.v = 0.f;
-        const bool occluded = traverse<true, COUNT>(j.sc, v3(o), v3(d), kEps, occ, st, cnt);
+        NeeSample ns;
+        ns.position = v3(o);
+        ns.normal = v3(d);
+        ns.mask = v3(m);
+        ns.li = __float_as_uint(m.w);
+        ns.lu = d.w;
+        ns.lv = e.x;
         ShadowReq sr;
-        sr.mask = v3(m);
-        sr.direct = v3(dr);
+        sr.direct = v3(0.f, 0.f, 0.f);
+        bool visible = false;
+        if (nee_prepare<COUNT>(j.sc, ns, sr, cnt)) {
+            HitRec occ;
+            occ.t = sr.tL;
+            occ.id = sr.lid;
+            occ.rec = 0;
+            occ.mat = 0;
+            occ.u = occ.v = 0.f;
+            visible = !traverse<true, COUNT>(j.sc, sr.so, sr.sd, kEps, occ, st, cnt);
+        }
         const float4 cl = a.pcolor[slot];
-        const V3 c = add_direct(v3(cl), sr, !occluded);
+        const V3 c = add_direct(v3(cl), ns.mask, sr.direct, visible);
         a.pcolor[slot] = make_float4(c.x, c.y, c.z, 0.f);
-        if (__float_as_uint(dr.w)) a.result[slot] = make_float4(c.x, c.y, c.z, 0.f);
+        if (__float_as_uint(e.y)) a.result[slot] = make_float4(c.x, c.y, c.z, 0.f);
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
@@ -418,8 +431,8 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     w.mem = nullptr;
     w.spill = nullptr;
     w.ctr = nullptr;
-    // per slot: 2 queues x 32 B, hit 16, mask 16, colour 16, info 16, shadow 64, result 16
-    const size_t per = 2 * 32 + 16 + 16 + 16 + 16 + 64 + 16;
+    // per slot: 2 queues x 32 B, hit 16 + 4, mask 16, colour 16, info 16, shadow 64, result 16
+    const size_t per = 2 * 32 + 16 + 4 + 16 + 16 + 16 + 64 + 16;
     w.bytes = per * size_t(capacity);
     WF_TRY(hipMalloc(&w.mem, w.bytes));
     WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.ctr), kNumCounters * sizeof(uint32_t)));
@@ -447,6 +460,7 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     a.sm = static_cast<float4*>(take(16 * C));
     a.sdir = static_cast<float4*>(take(16 * C));
     a.result = static_cast<float4*>(take(16 * C));
+    a.hmat = static_cast<uint32_t*>(take(4 * C));
     a.ctr = w.ctr;
     a.spill = w.spill;
     a.spill_stride = persistent;
