@@ -35,14 +35,19 @@
 namespace prt {
 
 constexpr int kMaxIter = 64;
+// Same-address atomics serialise (about 11 ns each on MI355X), so work is handed
+// out by kSpread counters, one 64-item batch per atomic, and appends to a queue are
+// aggregated per workgroup.
+constexpr uint32_t kSpread = 64;
+constexpr int kShadeBlock = 512;
 enum WfCounter : uint32_t {
     kExt0 = 0,      // ray count of queue 0
     kExt1 = 1,      // ray count of queue 1
     kShadow = 2,    // shadow ray count
     kChunk = 3,     // index of the chunk being rendered
     kPaths = 4,
-    kFetch = 8,     // per (kernel, iteration) fetch counters: 3 * kMaxIter
-    kStats = kFetch + 3 * kMaxIter,  // kCntNum 64-bit counters (two words each)
+    kFetch = 8,     // per (kernel, iteration) fetch counters: 3 * kMaxIter groups of kSpread
+    kStats = kFetch + 3 * kMaxIter * 64,  // kCntNum 64-bit counters (two words each)
     kNumCounters = kStats + 2 * kCntNum,
 };
 
@@ -65,22 +70,29 @@ struct WfChunk {
     uint32_t tile0, ntiles, paths, pixels;
 };
 
+// Queue entry (structure of arrays, one float4 each, coalesced by queue index):
+//   A  origin.xyz, pixel | job << 24
+//   B  direction.xyz, sample index
+//   C  throughput.xyz, previous bsdf
+//   D  colour.xyz, slot (result index)
+// Shadow entry: E position.xyz, target | F normal.xyz, lu | G mask.xyz, light record |
+//   H lv, final flag. `target` is the path's entry in the next queue, or its result
+//   slot when the path ended at this vertex.
 struct WfArgs {
     const WfJob* jobs;
     const WfTile* tiles;
     const WfChunk* chunks;
     uint32_t* ctr;
-    float4* qo[2];   // o.xyz, slot
-    float4* qd[2];   // d.xyz, 0
+    float4* qa[2];
+    float4* qb[2];
+    float4* qc[2];
+    float4* qd[2];
     float4* hits;    // t, u, v, record (0xFFFFFFFF = miss)
     uint32_t* hmat;  // material of the hit record
-    float4* pmask;   // mask.xyz, previous bsdf
-    float4* pcolor;  // color.xyz
-    uint4* pinfo;    // job, pixel, sample
-    float4* so;      // light sample: position.xyz, slot
-    float4* sd;      // normal.xyz, lu
-    float4* sm;      // mask.xyz, light record
-    float4* sdir;    // lv, final flag, 0, 0
+    float4* se;
+    float4* sf;
+    float4* sg;
+    float4* sh;
     float4* result;  // per slot final colour
     int* spill;
     uint32_t spill_stride;
@@ -105,21 +117,34 @@ struct WavefrontState {
 
 namespace dev {
 
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* fetch) {
-    uint32_t base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(fetch, 64u);
-    return __shfl(base, 0, 64);
-}
-
-// Appends the lanes with `pred` to a queue: returns this lane's position.
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* count) {
-    const unsigned long long m = __ballot(pred);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t n = uint32_t(__popcll(m));
-    uint32_t base = 0;
-    if (lane == 0 && n) base = atomicAdd(count, n);
-    base = __shfl(base, 0, 64);
-    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+// Hands out 64-item batches of [0, n). Batch b belongs to counter b % kSpread. A
+// wave starts on counter (wave % kSpread) and moves on when it is exhausted; a plain
+// load skips exhausted counters without an atomic. Returns the batch's first item
+// or 0xFFFFFFFF when every batch has been handed out.
+struct SpreadFetch {
+    uint32_t* ctr;
+    uint32_t k, tried, nb;
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t lane = threadIdx.x & 63;
+        while (tried < kSpread) {
+            uint32_t idx = 0xFFFFFFFFu;
+            if (lane == 0) {
+                const uint32_t limit = (nb + kSpread - 1 - k) / kSpread;  // batches of this counter
+                const uint32_t seen = __hip_atomic_load(ctr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen < limit) idx = atomicAdd(ctr + k, 1u);
+                if (idx >= limit) idx = 0xFFFFFFFFu;
+            }
+            idx = __shfl(idx, 0, 64);
+            if (idx != 0xFFFFFFFFu) return (idx * kSpread + k) * 64u;
+            k = (k + 1) % kSpread;
+            ++tried;
+        }
+        return 0xFFFFFFFFu;
+    }
+};
+__device__ __forceinline__ SpreadFetch make_fetch(uint32_t* ctr, uint32_t n) {
+    const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    return SpreadFetch{ctr, wave % kSpread, 0u, (n + 63) / 64};
 }
 
 template <bool COUNT>
@@ -149,7 +174,7 @@ __device__ __forceinline__ uint32_t find_tile(const WfTile* t, uint32_t first, u
 __global__ void wf_setup(WfArgs a) {
     if (threadIdx.x != 0) return;
     const uint32_t c = a.ctr[kChunk];
-    for (uint32_t k = 0; k < 3 * kMaxIter; ++k) a.ctr[kFetch + k] = 0;
+    for (uint32_t k = 0; k < 3 * kMaxIter * kSpread; ++k) a.ctr[kFetch + k] = 0;
     a.ctr[kExt0] = a.chunks[c].paths;
     a.ctr[kExt1] = 0;
     a.ctr[kShadow] = 0;
@@ -172,11 +197,22 @@ __global__ __launch_bounds__(256) void wf_generate(WfArgs a) {
         Rng rng{j.path.seed, j.path.view, pixel, sample};
         V3 ro, rd;
         camera_ray(j.cam, x, y, rng.block(0), ro, rd);
-        a.qo[0][slot] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(slot));
-        a.qd[0][slot] = make_float4(rd.x, rd.y, rd.z, 0.f);
-        a.pmask[slot] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
-        a.pcolor[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.pinfo[slot] = make_uint4(t.job, pixel, sample, 0u);
+        a.qa[0][slot] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(pixel | (t.job << 24)));
+        a.qb[0][slot] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(sample));
+        a.qc[0][slot] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
+        a.qd[0][slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(slot));
+    }
+}
+
+// Runs f(job) with the job index made wave-uniform (scalar loads of the job
+// descriptor) when every valid lane of the wave belongs to the same job.
+template <class F>
+__device__ __forceinline__ void with_job(bool valid, uint32_t job, F&& f) {
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(job);
+    if (__ballot(valid && job != j0) == 0ull) {
+        if (valid) f(j0);
+    } else {
+        if (valid) f(job);
     }
 }
 
@@ -196,57 +232,103 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
     st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
     st.sp = 0;
     Counts cnt = {0, 0, 0, 0, 0, 0};
+    SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it) * kSpread], n);
     while (true) {
-        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it]);
+        const uint32_t base = fetch.next();
         if (base >= n) break;
         const uint32_t i = base + lane;
-        if (i >= n) continue;
-        const float4 o = a.qo[q][i], d = a.qd[q][i];
-        const uint32_t slot = __float_as_uint(o.w);
-        const WfJob& j = a.jobs[a.pinfo[slot].x];
-        if (COUNT) {
-            if (it == 0) cnt.primary++; else cnt.extension++;
+        const bool valid = i < n;
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+        if (valid) {
+            o = a.qa[q][i];
+            d = a.qb[q][i];
         }
-        HitRec h;
-        h.t = kFltMax;
-        h.id = 0;
-        h.rec = 0xFFFFFFFFu;
-        h.mat = 0;
-        h.u = h.v = 0.f;
-        bool hit = traverse<false, COUNT>(j.sc, v3(o), v3(d), kEps, h, st, cnt);
-        a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
-        a.hmat[i] = h.mat;
+        const uint32_t job = __float_as_uint(o.w) >> 24;
+        with_job(valid, job, [&](uint32_t jb) {
+            const SceneArgs& sc = a.jobs[jb].sc;
+            if (COUNT) {
+                if (it == 0) cnt.primary++; else cnt.extension++;
+            }
+            HitRec h;
+            h.t = kFltMax;
+            h.id = 0;
+            h.rec = 0xFFFFFFFFu;
+            h.mat = 0;
+            h.u = h.v = 0.f;
+            bool hit = traverse<false, COUNT>(sc, v3(o), v3(d), kEps, h, st, cnt);
+            a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
+            a.hmat[i] = h.mat;
+        });
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
 
+// Appends to two queues for a whole workgroup with one atomic per queue: lanes
+// with more / nee get consecutive positions. Returns (next-queue, shadow) positions.
+__device__ __forceinline__ uint2 block_append(bool more, bool nee, uint32_t* ctr_next, uint32_t* ctr_shadow) {
+    __shared__ uint32_t s_off[2][kShadeBlock / 64];
+    __shared__ uint32_t s_base[2];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    const unsigned long long m0 = __ballot(more), m1 = __ballot(nee);
+    if (lane == 0) {
+        s_off[0][wave] = uint32_t(__popcll(m0));
+        s_off[1][wave] = uint32_t(__popcll(m1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t0 = 0, t1 = 0;
+        for (int w = 0; w < kShadeBlock / 64; ++w) {
+            uint32_t c0 = s_off[0][w], c1 = s_off[1][w];
+            s_off[0][w] = t0;
+            s_off[1][w] = t1;
+            t0 += c0;
+            t1 += c1;
+        }
+        s_base[0] = t0 ? atomicAdd(ctr_next, t0) : 0u;
+        s_base[1] = t1 ? atomicAdd(ctr_shadow, t1) : 0u;
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    uint2 r;
+    r.x = s_base[0] + s_off[0][wave] + uint32_t(__popcll(m0 & lt));
+    r.y = s_base[1] + s_off[1][wave] + uint32_t(__popcll(m1 & lt));
+    __syncthreads();  // s_off / s_base are reused by the next iteration
+    return r;
+}
+
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
+__global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
     const uint32_t q = it & 1u;
     const uint32_t n = a.ctr[kExt0 + q];
-    const uint32_t lane = threadIdx.x;
     Counts cnt = {0, 0, 0, 0, 0, 0};
-    // Static, wave-uniform partition of the queue (shading cost varies little).
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t i = base + lane;
+    // Static, workgroup-uniform partition of the queue (shading cost varies little).
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
         const bool valid = i < n;
         bool more = false, want_nee = false;
-        uint32_t slot = 0;
         PathState ps;
         NeeSample ns;
+        float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A, Cc = A, D = A, hh = A;
+        uint32_t hm = 0;
         if (valid) {
-            const float4 o = a.qo[q][i], d = a.qd[q][i], hh = a.hits[i];
-            slot = __float_as_uint(o.w);
-            const uint4 info = a.pinfo[slot];
-            const WfJob& j = a.jobs[info.x];
-            const float4 mk = a.pmask[slot], cl = a.pcolor[slot];
-            ps.ro = v3(o);
-            ps.rd = v3(d);
-            ps.mask = v3(mk);
-            ps.color = v3(cl);
-            ps.prev = __float_as_int(mk.w);
-            ps.b = it;
-            ps.input = it == 0 ? v3(0.f, 0.f, 0.f) : ps.rd;  // at b >= 1 the stale input is the ray direction
+            A = a.qa[q][i];
+            B = a.qb[q][i];
+            Cc = a.qc[q][i];
+            D = a.qd[q][i];
+            hh = a.hits[i];
+            hm = a.hmat[i];
+        }
+        const uint32_t pj = __float_as_uint(A.w);
+        const uint32_t slot = __float_as_uint(D.w);
+        ps.ro = v3(A);
+        ps.rd = v3(B);
+        ps.mask = v3(Cc);
+        ps.color = v3(D);
+        ps.prev = __float_as_int(Cc.w);
+        ps.b = it;
+        ps.input = it == 0 ? v3(0.f, 0.f, 0.f) : ps.rd;  // at b >= 1 the stale input is the ray direction
+        with_job(valid, pj >> 24, [&](uint32_t jb) {
+            const WfJob& j = a.jobs[jb];
             const uint32_t rec = __float_as_uint(hh.w);
             if (rec == 0xFFFFFFFFu) {
                 path_miss(ps);
@@ -257,25 +339,27 @@ __global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
                 h.v = hh.z;
                 h.rec = rec;
                 h.id = 0;
-                h.mat = a.hmat[i];
-                Rng rng{j.path.seed, j.path.view, info.y, info.z};
+                h.mat = hm;
+                Rng rng{j.path.seed, j.path.view, pj & 0xFFFFFFu, __float_as_uint(B.w)};
                 more = shade_vertex<COUNT, true>(j.sc, j.path, rng, ps, h, &want_nee, ns, cnt);
             }
-            a.pmask[slot] = make_float4(ps.mask.x, ps.mask.y, ps.mask.z, __int_as_float(ps.prev));
-            a.pcolor[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
-            if (!more && !want_nee) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
-        }
-        const uint32_t qi = wave_append(more, &a.ctr[kExt0 + (q ^ 1u)]);
+        });
+        if (valid && !more) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
+        const uint2 pos = block_append(more, want_nee, &a.ctr[kExt0 + (q ^ 1u)], &a.ctr[kShadow]);
+        const uint32_t qi = pos.x;
         if (more) {
-            a.qo[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, __uint_as_float(slot));
-            a.qd[q ^ 1u][qi] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, 0.f);
+            a.qa[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, A.w);
+            a.qb[q ^ 1u][qi] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, B.w);
+            a.qc[q ^ 1u][qi] = make_float4(ps.mask.x, ps.mask.y, ps.mask.z, __int_as_float(ps.prev));
+            a.qd[q ^ 1u][qi] = make_float4(ps.color.x, ps.color.y, ps.color.z, D.w);
         }
-        const uint32_t si = wave_append(want_nee, &a.ctr[kShadow]);
+        const uint32_t si = pos.y;
         if (want_nee) {
-            a.so[si] = make_float4(ns.position.x, ns.position.y, ns.position.z, __uint_as_float(slot));
-            a.sd[si] = make_float4(ns.normal.x, ns.normal.y, ns.normal.z, ns.lu);
-            a.sm[si] = make_float4(ns.mask.x, ns.mask.y, ns.mask.z, __uint_as_float(ns.li));
-            a.sdir[si] = make_float4(ns.lv, __uint_as_float(more ? 0u : 1u), 0.f, 0.f);
+            const uint32_t target = more ? qi : slot;
+            a.se[si] = make_float4(ns.position.x, ns.position.y, ns.position.z, __uint_as_float(target));
+            a.sf[si] = make_float4(ns.normal.x, ns.normal.y, ns.normal.z, ns.lu);
+            a.sg[si] = make_float4(ns.mask.x, ns.mask.y, ns.mask.z, __uint_as_float(ns.li));
+            a.sh[si] = make_float4(ns.lv, __uint_as_float(more ? 0u : 1u), __uint_as_float(pj >> 24), 0.f);
         }
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
@@ -284,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade(WfArgs a, uint32_t it) {
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     __shared__ int s_stack[kStackLdsQueue * kBlock];
+    const uint32_t q1 = (it & 1u) ^ 1u;  // the queue shade(it) filled
     const uint32_t n = a.ctr[kShadow];
     const uint32_t lane = threadIdx.x;
     StackT<kStackLdsQueue> st;
@@ -292,37 +377,46 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
     st.sp = 0;
     Counts cnt = {0, 0, 0, 0, 0, 0};
+    SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
     while (true) {
-        const uint32_t base = wave_fetch(&a.ctr[kFetch + 3 * it + 2]);
+        const uint32_t base = fetch.next();
         if (base >= n) break;
         const uint32_t i = base + lane;
-        if (i >= n) continue;
-        const float4 o = a.so[i], d = a.sd[i], m = a.sm[i], e = a.sdir[i];
-        const uint32_t slot = __float_as_uint(o.w);
-        const WfJob& j = a.jobs[a.pinfo[slot].x];
-        NeeSample ns;
-        ns.position = v3(o);
-        ns.normal = v3(d);
-        ns.mask = v3(m);
-        ns.li = __float_as_uint(m.w);
-        ns.lu = d.w;
-        ns.lv = e.x;
-        ShadowReq sr;
-        sr.direct = v3(0.f, 0.f, 0.f);
-        bool visible = false;
-        if (nee_prepare<COUNT>(j.sc, ns, sr, cnt)) {
-            HitRec occ;
-            occ.t = sr.tL;
-            occ.id = sr.lid;
-            occ.rec = 0;
-            occ.mat = 0;
-            occ.u = occ.v = 0.f;
-            visible = !traverse<true, COUNT>(j.sc, sr.so, sr.sd, kEps, occ, st, cnt);
+        const bool valid = i < n;
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f), f = e, g = e, hx = e;
+        if (valid) {
+            e = a.se[i];
+            f = a.sf[i];
+            g = a.sg[i];
+            hx = a.sh[i];
         }
-        const float4 cl = a.pcolor[slot];
-        const V3 c = add_direct(v3(cl), ns.mask, sr.direct, visible);
-        a.pcolor[slot] = make_float4(c.x, c.y, c.z, 0.f);
-        if (__float_as_uint(e.y)) a.result[slot] = make_float4(c.x, c.y, c.z, 0.f);
+        with_job(valid, __float_as_uint(hx.z), [&](uint32_t jb) {
+            const SceneArgs& sc = a.jobs[jb].sc;
+            NeeSample ns;
+            ns.position = v3(e);
+            ns.normal = v3(f);
+            ns.mask = v3(g);
+            ns.li = __float_as_uint(g.w);
+            ns.lu = f.w;
+            ns.lv = hx.x;
+            ShadowReq sr;
+            sr.direct = v3(0.f, 0.f, 0.f);
+            bool visible = false;
+            if (nee_prepare<COUNT>(sc, ns, sr, cnt)) {
+                HitRec occ;
+                occ.t = sr.tL;
+                occ.id = sr.lid;
+                occ.rec = 0;
+                occ.mat = 0;
+                occ.u = occ.v = 0.f;
+                visible = !traverse<true, COUNT>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
+            }
+            const uint32_t target = __float_as_uint(e.w);
+            float4* dst = __float_as_uint(hx.y) ? a.result + target : a.qd[q1] + target;
+            const float4 c = *dst;
+            const V3 s = add_direct(v3(c), ns.mask, sr.direct, visible);
+            *dst = make_float4(s.x, s.y, s.z, c.w);
+        });
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
@@ -431,8 +525,8 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     w.mem = nullptr;
     w.spill = nullptr;
     w.ctr = nullptr;
-    // per slot: 2 queues x 32 B, hit 16 + 4, mask 16, colour 16, info 16, shadow 64, result 16
-    const size_t per = 2 * 32 + 16 + 4 + 16 + 16 + 16 + 64 + 16;
+    // per slot: 2 queues x 64 B, hit 16 + 4, shadow 64, result 16
+    const size_t per = 2 * 64 + 16 + 4 + 64 + 16;
     w.bytes = per * size_t(capacity);
     WF_TRY(hipMalloc(&w.mem, w.bytes));
     WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.ctr), kNumCounters * sizeof(uint32_t)));
@@ -447,18 +541,17 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     };
     const size_t C = capacity;
     WfArgs& a = w.args;
-    a.qo[0] = static_cast<float4*>(take(16 * C));
-    a.qd[0] = static_cast<float4*>(take(16 * C));
-    a.qo[1] = static_cast<float4*>(take(16 * C));
-    a.qd[1] = static_cast<float4*>(take(16 * C));
+    for (int k = 0; k < 2; ++k) {
+        a.qa[k] = static_cast<float4*>(take(16 * C));
+        a.qb[k] = static_cast<float4*>(take(16 * C));
+        a.qc[k] = static_cast<float4*>(take(16 * C));
+        a.qd[k] = static_cast<float4*>(take(16 * C));
+    }
     a.hits = static_cast<float4*>(take(16 * C));
-    a.pmask = static_cast<float4*>(take(16 * C));
-    a.pcolor = static_cast<float4*>(take(16 * C));
-    a.pinfo = static_cast<uint4*>(take(16 * C));
-    a.so = static_cast<float4*>(take(16 * C));
-    a.sd = static_cast<float4*>(take(16 * C));
-    a.sm = static_cast<float4*>(take(16 * C));
-    a.sdir = static_cast<float4*>(take(16 * C));
+    a.se = static_cast<float4*>(take(16 * C));
+    a.sf = static_cast<float4*>(take(16 * C));
+    a.sg = static_cast<float4*>(take(16 * C));
+    a.sh = static_cast<float4*>(take(16 * C));
     a.result = static_cast<float4*>(take(16 * C));
     a.hmat = static_cast<uint32_t*>(take(4 * C));
     a.ctr = w.ctr;
@@ -487,6 +580,10 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
                   const std::vector<PlanJob>& pjobs, uint32_t max_depth, uint32_t capacity, uint32_t blocks_per_cu,
                   bool count, RunStats* rs) {
     if (max_depth == 0 || max_depth > uint32_t(kMaxIter)) return fail(PRT_ERR_INVALID_ARG, "max_depth out of range");
+    // Queue entries pack pixel | job << 24.
+    if (jobs.size() > 256) return fail(PRT_ERR_LIMIT, "more than 256 jobs in one batch");
+    for (const PlanJob& pj : pjobs)
+        if (pj.pixels > (1u << 24)) return fail(PRT_ERR_LIMIT, "a view with more than 2^24 pixels");
     std::vector<WfTile> tiles;
     std::vector<WfChunk> chunks;
     make_plan(pjobs, capacity, tiles, chunks);
@@ -516,6 +613,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     hipDeviceProp_t prop;
     WF_TRY(hipGetDeviceProperties(&prop, device));
     const int gblocks = prop.multiProcessorCount * 8;
+    const int sblocks = prop.multiProcessorCount * 2;  // 2 x 8 waves of shade per compute unit
 
     // Capture one chunk's launch sequence.
     hipGraph_t graph = nullptr;
@@ -526,11 +624,11 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     for (uint32_t it = 0; it < max_depth; ++it) {
         if (count) {
             hipLaunchKernelGGL(dev::wf_extend<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shade<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shade<true>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
             hipLaunchKernelGGL(dev::wf_shadow<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
         } else {
             hipLaunchKernelGGL(dev::wf_extend<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shade<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+            hipLaunchKernelGGL(dev::wf_shade<false>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
             hipLaunchKernelGGL(dev::wf_shadow<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
         }
     }
