@@ -118,33 +118,41 @@ struct WavefrontState {
 namespace dev {
 
 // Hands out 64-item batches of [0, n). Batch b belongs to counter b % kSpread. A
-// wave starts on counter (wave % kSpread) and moves on when it is exhausted; a plain
-// load skips exhausted counters without an atomic. Returns the batch's first item
-// or 0xFFFFFFFF when every batch has been handed out.
+// wave draws from counter (wave % kSpread); issue() sends the atomic for its next
+// batch before the current batch is processed, so the round trip overlaps the
+// traversal. When its counter is exhausted the wave probes the others, with a
+// coherent load first so exhausted counters cost no atomic. resolve() returns the
+// batch's first item or 0xFFFFFFFF when every batch has been handed out.
 struct SpreadFetch {
     uint32_t* ctr;
-    uint32_t k, tried, nb;
-    __device__ __forceinline__ uint32_t next() {
-        const uint32_t lane = threadIdx.x & 63;
-        while (tried < kSpread) {
-            uint32_t idx = 0xFFFFFFFFu;
-            if (lane == 0) {
-                const uint32_t limit = (nb + kSpread - 1 - k) / kSpread;  // batches of this counter
+    uint32_t k, nb;
+    __device__ __forceinline__ uint32_t limit(uint32_t c) const { return (nb + kSpread - 1 - c) / kSpread; }
+    __device__ __forceinline__ uint32_t issue() const {
+        uint32_t idx = 0xFFFFFFFFu;
+        if ((threadIdx.x & 63) == 0) idx = atomicAdd(ctr + k, 1u);
+        return idx;
+    }
+    __device__ __forceinline__ uint32_t resolve(uint32_t raw) {
+        uint32_t idx = __shfl(raw, 0, 64);
+        if (idx < limit(k)) return (idx * kSpread + k) * 64u;
+        for (uint32_t probe = 1; probe < kSpread; ++probe) {
+            k = (k + 1) % kSpread;
+            idx = 0xFFFFFFFFu;
+            if ((threadIdx.x & 63) == 0) {
+                const uint32_t lim = limit(k);
                 const uint32_t seen = __hip_atomic_load(ctr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (seen < limit) idx = atomicAdd(ctr + k, 1u);
-                if (idx >= limit) idx = 0xFFFFFFFFu;
+                if (seen < lim) idx = atomicAdd(ctr + k, 1u);
+                if (idx >= lim) idx = 0xFFFFFFFFu;
             }
             idx = __shfl(idx, 0, 64);
             if (idx != 0xFFFFFFFFu) return (idx * kSpread + k) * 64u;
-            k = (k + 1) % kSpread;
-            ++tried;
         }
         return 0xFFFFFFFFu;
     }
 };
 __device__ __forceinline__ SpreadFetch make_fetch(uint32_t* ctr, uint32_t n) {
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    return SpreadFetch{ctr, wave % kSpread, 0u, (n + 63) / 64};
+    return SpreadFetch{ctr, wave % kSpread, (n + 63) / 64};
 }
 
 template <bool COUNT>
@@ -233,9 +241,9 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
     st.sp = 0;
     Counts cnt = {0, 0, 0, 0, 0, 0};
     SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it) * kSpread], n);
-    while (true) {
-        const uint32_t base = fetch.next();
-        if (base >= n) break;
+    uint32_t base = n ? fetch.resolve(fetch.issue()) : 0xFFFFFFFFu;
+    while (base < n) {
+        const uint32_t pending = fetch.issue();
         const uint32_t i = base + lane;
         const bool valid = i < n;
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
@@ -259,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
             a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
             a.hmat[i] = h.mat;
         });
+        base = fetch.resolve(pending);
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
@@ -378,9 +387,9 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     st.sp = 0;
     Counts cnt = {0, 0, 0, 0, 0, 0};
     SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
-    while (true) {
-        const uint32_t base = fetch.next();
-        if (base >= n) break;
+    uint32_t base = n ? fetch.resolve(fetch.issue()) : 0xFFFFFFFFu;
+    while (base < n) {
+        const uint32_t pending = fetch.issue();
         const uint32_t i = base + lane;
         const bool valid = i < n;
         float4 e = make_float4(0.f, 0.f, 0.f, 0.f), f = e, g = e, hx = e;
@@ -417,6 +426,7 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
             const V3 s = add_direct(v3(c), ns.mask, sr.direct, visible);
             *dst = make_float4(s.x, s.y, s.z, c.w);
         });
+        base = fetch.resolve(pending);
     }
     if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
 }
