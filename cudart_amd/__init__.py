@@ -275,6 +275,17 @@ class ParallelRenderer:
                                 _capi.fptr(d_alb), _capi.fptr(d_em), C.byref(loss)))
         return d_alb, d_em, loss.value
 
+    def adjoint_grad(self, grad_image: np.ndarray, spp: int, hess: float = 0.0):
+        """Path-replay gradients of a loss with dLoss/dI = grad_image and constant diagonal
+        curvature d2Loss/dI2 = hess w.r.t. kd (M,3) and emission (M,)."""
+        M = self.scene.num_materials
+        d_alb = np.zeros((M, 3), np.float32)
+        d_em = np.zeros(M, np.float32)
+        g = np.ascontiguousarray(grad_image, np.float32)
+        check(lib().prt_adjoint_grad(self._h, _capi.fptr(g), float(hess), spp, _capi.fptr(d_alb),
+                                     _capi.fptr(d_em)))
+        return d_alb, d_em
+
     def close(self) -> None:
         if self._h:
             lib().prt_renderer_destroy(self._h)

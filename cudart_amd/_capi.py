@@ -117,6 +117,7 @@ SIGNATURES = {
     "prt_render_batch": (I, [C.POINTER(Job), U, C.POINTER(RenderConfig), C.POINTER(Tune), C.POINTER(Stats)]),
     "prt_autotune": (I, [C.POINTER(Job), U, C.POINTER(RenderConfig), C.POINTER(Tune), C.POINTER(C.c_double)]),
     "prt_adjoint": (I, [P, FP, FP, U, FP, FP, FP]),
+    "prt_adjoint_grad": (I, [P, FP, C.c_float, U, FP, FP]),
     "prt_write_png": (I, [C.c_char_p, U, U, BP]),
     "prt_write_pfm": (I, [C.c_char_p, U, U, FP]),
 }

@@ -40,7 +40,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
 
 HOST_SOURCES = ["host/errors.cpp", "host/mesh_io.cpp", "host/procgen.cpp", "host/bvh.cpp", "host/scene.cpp",
                 "host/image_io.cpp"]
-HIP_SOURCES = ["kernels/render.hip", "kernels/device.hip", "kernels/wavefront.hip"]
+HIP_SOURCES = ["kernels/render.hip", "kernels/device.hip", "kernels/wavefront.hip", "kernels/adjoint.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -288,7 +288,7 @@ int gen_cornell_spheres(const Params& P, uint32_t seed, prt_mesh& m) {
     add_quad(m, F3{-S, 0, -S}, F3{S, 0, -S}, F3{S, 2 * S, -S}, F3{-S, 2 * S, -S}, F3{0, 0, 1}, 0);  // back
     add_quad(m, F3{-S, 0, -S}, F3{-S, 2 * S, -S}, F3{-S, 2 * S, S}, F3{-S, 0, S}, F3{1, 0, 0}, 1);  // left (red)
     add_quad(m, F3{S, 0, -S}, F3{S, 0, S}, F3{S, 2 * S, S}, F3{S, 2 * S, -S}, F3{-1, 0, 0}, 2);    // right (green)
-    const float ly = 2 * S - 0.01f, L = 1.25f;
+    const float ly = 2 * S - 0.5f, L = 1.25f;  // half a unit below the ceiling: no near-coplanar light
     add_quad(m, F3{-L, ly, -L}, F3{L, ly, -L}, F3{L, ly, L}, F3{-L, ly, L}, F3{0, -1, 0}, 3);
     Sphere s = icosphere(level);
     for (int i = 0; i < instances; ++i) {

@@ -1,0 +1,363 @@
+// Path-replay backpropagation (Vicini, Speierer, Jakob: "Path Replay
+// Backpropagation: Differentiating Light Paths using Constant Memory and Linear
+// Time", SIGGRAPH 2021) for the reference's estimator.
+//
+// Given dLoss/dI for every pixel (I = mean of `spp` samples), each sample's
+// radiance L has adjoint delta = dLoss/dI / spp. For each pixel-sample the kernel
+// traces the path once (primal, the
+// megakernel's arithmetic) to get its radiance L, then replays it with the same
+// Philox slots, keeping the radiance still to be collected, rem = L - (what the
+// path collected so far). Parameters:
+//   kd_m (RGB)  a diffuse vertex multiplies everything collected after it by
+//               kd_c, so it contributes delta_c * rem_c / kd_c (rem taken after
+//               that vertex's emission, before its next-event estimate);
+//   e_m         MTL Ka = e_m (1,1,1), ka = 3 e_m A_L: emission terms contribute
+//               delta . mask * 3 A_L / A_L and next-event terms delta . mask * 3 A_L wf.
+// Russian roulette (p = max(mask) after the vertex's weight) is differentiated
+// as well: where p >= 1 the path always survives and mask /= p is a
+// deterministic rescale; where p < 1 the survival probability depends on the
+// albedos, and the boundary (score) term of that decision is weighted by the
+// loss change it causes, which for a loss with curvature hess includes
+// -(hess / 2) |R / spp|^2 (R = radiance collected after the decision). With
+// both, the result is an unbiased estimate of the derivative of the expected
+// loss, the quantity a central difference of renders with common random
+// numbers estimates. Textured albedos get no kd gradient.
+// Gradients accumulate in fp64: LDS per workgroup, then a fixed-order reduction.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../host/common.hpp"
+#include "path.hpp"
+#include "renderer.hpp"
+
+namespace prt {
+
+struct AdjArgs {
+    SceneArgs sc;
+    CamArgs cam;
+    PathArgs path;
+    uint32_t width, height, spp;
+    uint32_t tiles_x, tiles;
+    const float* grad;    // W*H*3 dLoss / dI
+    double norm;          // 1 / spp: dI / dL of one sample
+    double half_hess;     // (d2 Loss / dI2) / (2 spp^2): loss curvature per sample radiance
+    uint32_t num_params;  // 4 * materials: kd (M*3) then emission (M)
+    uint32_t num_materials;
+    double* partial;      // grid x num_params
+    int* spill;
+    uint32_t spill_stride;
+};
+
+namespace dev {
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3(V3 v) { return D3{double(v.x), double(v.y), double(v.z)}; }
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// Tie-aware weight of each channel in d log max(mask): 1 for a unique maximum,
+// 1/2 for each of two or three equal maxima (the mean of the one-sided
+// derivatives of max, which is what a central difference measures).
+__device__ __forceinline__ D3 rr_weight(uint32_t ties) {
+    const double w = __popc(ties) == 1 ? 1.0 : 0.5;
+    return D3{(ties & 1u) ? w : 0.0, (ties & 2u) ? w : 0.0, (ties & 4u) ? w : 0.0};
+}
+
+// Traces the shadow ray of a next-event sample; true when the light is visible.
+template <class StackType>
+__device__ __forceinline__ bool shadow_visible(const SceneArgs& sc, const NeeSample& ns, ShadowReq& sr, StackType& st,
+                                               Counts& cnt) {
+    sr.direct = v3(0.f, 0.f, 0.f);
+    if (!nee_prepare<false>(sc, ns, sr, cnt)) return false;
+    HitRec occ;
+    occ.t = sr.tL;
+    occ.id = sr.lid;
+    occ.rec = 0;
+    occ.mat = 0;
+    occ.u = occ.v = 0.f;
+    return !traverse<true, false>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
+}
+
+// samplePixel's radiance (the megakernel's arithmetic).
+template <class StackType>
+__device__ V3 primal(const AdjArgs& a, int x, int y, uint32_t sample, StackType& st, Counts& cnt) {
+    const SceneArgs& sc = a.sc;
+    Rng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
+    V3 ro, rd;
+    camera_ray(a.cam, x, y, rng.block(0), ro, rd);
+    PathState ps;
+    path_init(ps, ro, rd);
+    while (true) {
+        HitRec h;
+        h.t = kFltMax;
+        h.id = 0;
+        h.rec = 0;
+        h.mat = 0;
+        h.u = h.v = 0.f;
+        if (!traverse<false, false>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) {
+            path_miss(ps);
+            break;
+        }
+        bool want_nee;
+        NeeSample ns;
+        const bool more = shade_vertex<false, false>(sc, a.path, rng, ps, h, &want_nee, ns, cnt);
+        if (want_nee) {
+            ShadowReq sr;
+            const bool visible = shadow_visible(sc, ns, sr, st, cnt);
+            ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
+        }
+        if (!more) break;
+    }
+    return ps.color;
+}
+
+// Materials of the path's untextured diffuse vertices, 8 bits each (at most 16).
+struct VertexList {
+    uint64_t lo = 0, hi = 0;
+    uint32_t n = 0;
+    __device__ __forceinline__ void push(uint32_t m) {
+        if (n < 8) lo |= uint64_t(m) << (8 * n);
+        else hi |= uint64_t(m) << (8 * (n - 8));
+        ++n;
+    }
+    __device__ __forceinline__ uint32_t at(uint32_t i) const {
+        return uint32_t(((i < 8 ? lo : hi) >> (8 * (i & 7))) & 0xffu);
+    }
+};
+
+// The replay. L comes from the primal pass of the same sample.
+template <class StackType>
+__device__ void replay(const AdjArgs& a, int x, int y, uint32_t sample, D3 delta, V3 L, double* g, StackType& st,
+                       Counts& cnt) {
+    const SceneArgs& sc = a.sc;
+    const uint32_t M = a.num_materials;
+    const double gain = 3.0 * double(sc.lights_area);  // d ka / d e
+    D3 rem = d3(L);
+    VertexList vl;
+    uint32_t prev_ties = 0;  // channels at the maximum at the previous roulette step (0: none)
+    Rng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
+    V3 ro, rd;
+    camera_ray(a.cam, x, y, rng.block(0), ro, rd);
+    PathState ps;
+    path_init(ps, ro, rd);
+    while (true) {
+        HitRec h;
+        h.t = kFltMax;
+        h.id = 0;
+        h.rec = 0;
+        h.mat = 0;
+        h.u = h.v = 0.f;
+        if (!traverse<false, false>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) break;  // sky: no parameter
+        const float4 mkd = sc.mats[size_t(h.mat) * 4], mka = sc.mats[size_t(h.mat) * 4 + 1];
+        if (ps.b == 0 || ps.prev == 1 || ps.prev == 4) {
+            // the emission term shade_vertex is about to add
+            const V3 e = ps.mask * v3(mka) * sc.inv_lights_area;
+            atomicAdd(&g[3 * M + h.mat], ddot(delta, d3(ps.mask)) * gain * double(sc.inv_lights_area));
+            rem = dsub(rem, d3(e));
+        }
+        bool want_nee;
+        NeeSample ns;
+        RrInfo rr;
+        const uint32_t b = ps.b;
+        const bool more = shade_vertex<false, false>(sc, a.path, rng, ps, h, &want_nee, ns, cnt, &rr);
+        const uint32_t before = vl.n;  // diffuse vertices ahead of this one
+        if (ps.prev == 0 /*DIFFUSE*/ && __float_as_int(mka.w) < 0) {
+            const double kd[3] = {mkd.x, mkd.y, mkd.z};
+            const double rc[3] = {rem.x, rem.y, rem.z};
+            const double dc[3] = {delta.x, delta.y, delta.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (kd[c] != 0.0) atomicAdd(&g[3 * h.mat + c], dc[c] * rc[c] / kd[c]);
+            vl.push(h.mat);
+        }
+        if (want_nee) {
+            ShadowReq sr;
+            const bool visible = shadow_visible(sc, ns, sr, st, cnt);
+            if (visible) {
+                rem = dsub(rem, d3(ns.mask * sr.direct));
+                atomicAdd(&g[3 * M + sr.lmat], ddot(delta, d3(ns.mask)) * gain * double(sr.wf));
+            }
+            ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
+        }
+        if (rr.p >= 0.f && ps.b == b + 1) {
+            // Survived roulette step j. Everything collected after it, R_j = L - C_j,
+            // carries 1/p_j, and with S_k[c] = sum over diffuse vertices v <= k of
+            // d log kd_{m_v,c}: d log p_j = w_j . S_j - w_{j-1} . S_{j-1}. The step adds
+            // -kappa_j d log p_j, with kappa_j = delta . R_j where p_j >= 1 (the path
+            // always survives; mask /= p_j is a deterministic rescale), and where
+            // p_j < 1 kappa_j = (hess / 2) |R_j / spp|^2: the score term of the survival
+            // probability times the loss change of that decision, delta . R_j -
+            // (hess / 2) |R_j / spp|^2, less the derivative of 1/p_j.
+            const D3 R = dsub(d3(L), d3(ps.color));
+            const double kappa = rr.p >= 1.f ? ddot(delta, R) : a.half_hess * ddot(R, R);
+            if (kappa != 0.0) {
+                const D3 w = rr_weight(rr.ties), wp = rr_weight(prev_ties);
+                for (uint32_t i = 0; i < vl.n; ++i) {
+                    const uint32_t m = vl.at(i);
+                    const float4 k4 = sc.mats[size_t(m) * 4];
+                    const double kd[3] = {k4.x, k4.y, k4.z};
+                    const double cf[3] = {i < before ? wp.x - w.x : -w.x, i < before ? wp.y - w.y : -w.y,
+                                          i < before ? wp.z - w.z : -w.z};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        if (cf[c] != 0.0 && kd[c] != 0.0) atomicAdd(&g[3 * m + c], kappa * cf[c] / kd[c]);
+                }
+            }
+            prev_ties = rr.ties;
+        }
+        if (!more) break;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
+    __shared__ int s_stack[kStackLds * kBlock];
+    __shared__ double s_g[4 * PRT_MAX_MATERIALS];
+    for (uint32_t k = threadIdx.x; k < a.num_params; k += kBlock) s_g[k] = 0.0;
+    __syncthreads();
+    const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
+    const int lane = int(threadIdx.x);
+    if (tile < a.tiles) {
+        const int x = int(tile % a.tiles_x) * 8 + (lane & 7);
+        const int y = int(tile / a.tiles_x) * 8 + (lane >> 3);
+        if (x < int(a.width) && y < int(a.height)) {
+            Stack st;
+            st.lds = s_stack + lane;
+            st.spill_stride = a.spill_stride;
+            st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
+            st.sp = 0;
+            Counts cnt = {0, 0, 0, 0, 0, 0};
+            const size_t idx = (size_t(y) * a.width + size_t(x)) * 3;
+            const D3 delta{a.norm * double(a.grad[idx]), a.norm * double(a.grad[idx + 1]),
+                           a.norm * double(a.grad[idx + 2])};
+            for (uint32_t s = 0; s < a.spp; ++s) {
+                const V3 L = primal(a, x, y, s, st, cnt);
+                replay(a, x, y, s, delta, L, s_g, st, cnt);
+            }
+        }
+    }
+    __syncthreads();
+    double* out = a.partial + size_t(blockIdx.x) * a.num_params;
+    for (uint32_t k = threadIdx.x; k < a.num_params; k += kBlock) out[k] = s_g[k];
+}
+
+// Sums the per-workgroup partials of every parameter in workgroup order.
+__global__ void adjoint_reduce(const double* partial, uint32_t blocks, uint32_t num_params, double* out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= num_params) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < blocks; ++b) s += partial[size_t(b) * num_params + k];
+    out[k] = s;
+}
+
+}  // namespace dev
+
+namespace {
+#define ADJ_TRY(expr)                                           \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) {                                 \
+            rc = hip_fail(e_, #expr);                           \
+            goto done;                                          \
+        }                                                       \
+    } while (0)
+}  // namespace
+
+}  // namespace prt
+
+using namespace prt;
+
+extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float hess, uint32_t spp, float* d_albedo,
+                                float* d_emission) {
+    clear_error();
+    if (!r || !grad_rgb || !d_albedo || !d_emission || spp == 0 || !(hess >= 0.f))
+        return fail(PRT_ERR_INVALID_ARG, "bad argument");
+    if (r->cfg.max_depth > 16) return fail(PRT_ERR_LIMIT, "the adjoint supports max_depth <= 16");
+    int rc = set_device(r->cfg.device);
+    if (rc != PRT_OK) return rc;
+    rc = device_scene(r->scene, r->cfg.device, r->sc);
+    if (rc != PRT_OK) return rc;
+    const size_t px = size_t(r->w) * r->h;
+    const uint32_t M = r->scene->num_materials;
+    const uint32_t P = 4 * M;
+    const uint32_t grid = ((r->tiles + 7) / 8) * 8;
+    float* d_grad = nullptr;
+    double* d_part = nullptr;
+    double* d_out = nullptr;
+    std::vector<double> g(P);
+    float ms = 0.f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AdjArgs a;
+    ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_grad), px * 3 * sizeof(float)));
+    ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_part), size_t(grid) * P * sizeof(double)));
+    ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_out), P * sizeof(double)));
+    ADJ_TRY(hipMemcpy(d_grad, grad_rgb, px * 3 * sizeof(float), hipMemcpyHostToDevice));
+    a.sc = r->sc;
+    a.cam = cam_args(r->cam);
+    a.path = PathArgs{r->cfg.seed, r->cfg.view_id, r->cfg.max_depth, r->cfg.rr_start};
+    a.width = r->w;
+    a.height = r->h;
+    a.spp = spp;
+    a.tiles_x = r->tiles_x;
+    a.tiles = r->tiles;
+    a.grad = d_grad;
+    a.norm = 1.0 / double(spp);
+    a.half_hess = double(hess) / (2.0 * double(spp) * double(spp));
+    a.num_params = P;
+    a.num_materials = M;
+    a.partial = d_part;
+    a.spill = r->spill;
+    a.spill_stride = r->spill_threads;
+    ADJ_TRY(hipEventCreate(&e0));
+    ADJ_TRY(hipEventCreate(&e1));
+    ADJ_TRY(hipEventRecord(e0, r->stream));
+    hipLaunchKernelGGL(dev::adjoint_kernel, dim3(grid), dim3(kBlock), 0, r->stream, a);
+    ADJ_TRY(hipGetLastError());
+    hipLaunchKernelGGL(dev::adjoint_reduce, dim3((P + 255) / 256), dim3(256), 0, r->stream, d_part, grid, P, d_out);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipEventRecord(e1, r->stream));
+    ADJ_TRY(hipEventSynchronize(e1));
+    ADJ_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ADJ_TRY(hipMemcpy(g.data(), d_out, P * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < 3 * M; ++k) d_albedo[k] = float(g[k]);
+    for (uint32_t k = 0; k < M; ++k) d_emission[k] = float(g[3 * M + k]);
+    r->last_ms = ms;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    void* ptrs[] = {d_grad, d_part, d_out};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+extern "C" int prt_adjoint(prt_renderer* r, const float* target_rgb, const float* image_rgb, uint32_t spp,
+                           float* d_albedo, float* d_emission, float* loss) {
+    clear_error();
+    if (!r || !target_rgb || !d_albedo || !d_emission || spp == 0) return fail(PRT_ERR_INVALID_ARG, "bad argument");
+    const size_t n = size_t(r->w) * r->h * 3;
+    std::vector<float> img(n), grad(n);
+    int rc = PRT_OK;
+    if (image_rgb) {
+        std::memcpy(img.data(), image_rgb, n * sizeof(float));
+    } else {
+        // Forward pass at the same seed: the renderer's accumulation is replaced.
+        rc = prt_renderer_reset(r);
+        if (rc == PRT_OK) rc = prt_render_spp(r, spp);
+        if (rc == PRT_OK) rc = prt_get_radiance(r, img.data());
+        if (rc != PRT_OK) return rc;
+    }
+    // d/dI of sum (I - T)^2 / n
+    double l = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double d = double(img[k]) - double(target_rgb[k]);
+        l += d * d;
+        grad[k] = float(2.0 * d / double(n));
+    }
+    if (loss) *loss = float(l / double(n));
+    return prt_adjoint_grad(r, grad.data(), float(2.0 / double(n)), spp, d_albedo, d_emission);
+}

@@ -38,6 +38,8 @@ struct ShadowReq {
     float tL;        // distance to the light sample (the occlusion bound)
     uint32_t lid;    // original id of the light triangle (ties)
     V3 direct;       // 0 + lightColor * wf, the value when the light is visible
+    float wf;        // geometric weight A_l cos / (d^2 A_L)
+    uint32_t lmat;   // material of the light triangle
 };
 
 __device__ __forceinline__ void path_init(PathState& ps, V3 ro, V3 rd) {
@@ -59,6 +61,13 @@ __device__ __forceinline__ V3 add_direct(V3 color, V3 mask, V3 direct, bool visi
     return color + mask * (visible ? direct : v3(0.f, 0.f, 0.f));
 }
 
+// Russian-roulette step of one vertex, reported to the adjoint: p = max(mask)
+// before the division, and the channels that attain it (bit c set).
+struct RrInfo {
+    float p;       // < 0: no roulette at this vertex
+    uint32_t ties;
+};
+
 // Shades the hit `h` of the current ray. Returns true when the path continues
 // (next ray in ps.ro / ps.rd, bounce incremented); sets *want_nee and fills `ns`
 // at a diffuse vertex whose chosen light is another triangle.
@@ -68,8 +77,10 @@ __device__ __forceinline__ V3 add_direct(V3 color, V3 mask, V3 direct, bool visi
 // are the same either way: a block depends only on (pixel, sample, bounce).
 template <bool COUNT, bool EAGER>
 __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs& pa, const Rng& rng, PathState& ps,
-                                             const HitRec& h, bool* want_nee, NeeSample& ns, Counts& cnt) {
+                                             const HitRec& h, bool* want_nee, NeeSample& ns, Counts& cnt,
+                                             RrInfo* rr = nullptr) {
     *want_nee = false;
+    if (rr) rr->p = -1.f;
     const uint32_t b = ps.b;
     uint4 ra = make_uint4(0, 0, 0, 0), rb = make_uint4(0, 0, 0, 0);
     if (EAGER) {
@@ -202,6 +213,10 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
     ps.rd = ps.input;
     if (b >= pa.rr_start) {
         float p = max_component(ps.mask);
+        if (rr) {
+            rr->p = p;
+            rr->ties = (ps.mask.x == p ? 1u : 0u) | (ps.mask.y == p ? 2u : 0u) | (ps.mask.z == p ? 4u : 0u);
+        }
         if (u01(rb.z) > p) return false;
         ps.mask = ps.mask / p;
     }
@@ -237,6 +252,8 @@ __device__ __forceinline__ bool nee_prepare(const SceneArgs& sc, const NeeSample
     sr.tL = tL;
     sr.lid = __float_as_uint(la.w);
     sr.direct = v3(0.f, 0.f, 0.f) + v3(lka) * wf;
+    sr.wf = wf;
+    sr.lmat = __float_as_uint(lb.w);
     return true;
 }
 

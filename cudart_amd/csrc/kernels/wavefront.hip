@@ -875,6 +875,3 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     PRT_TRY_END
 }
 
-extern "C" int prt_adjoint(prt_renderer*, const float*, const float*, uint32_t, float*, float*, float*) {
-    return prt::fail(PRT_ERR_UNSUPPORTED, "adjoint not built");
-}

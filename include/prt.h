@@ -282,6 +282,16 @@ int prt_autotune(prt_job *jobs, uint32_t num_jobs, const prt_render_config *cfg,
 int prt_adjoint(prt_renderer *r, const float *target_rgb, const float *image_rgb, uint32_t spp,
                 float *d_albedo, float *d_emission, float *loss);
 
+/* The same for an arbitrary upstream gradient: grad_rgb = dLoss/dI (host, W*H*3) of the
+ * mean-radiance image of `spp` samples at the renderer's seed; the backward pass of an
+ * autograd function. hess = d2Loss/dI2 of one image element, taken as constant and
+ * diagonal (2/(W*H*3) for the mean squared error; 0 for a first-order gradient): it
+ * weights the Russian-roulette decisions, whose survival probability depends on the
+ * albedos and which change the loss by a finite step. d_albedo: M*3, d_emission: M
+ * (overwritten). Supports max_depth <= 16. */
+int prt_adjoint_grad(prt_renderer *r, const float *grad_rgb, float hess, uint32_t spp, float *d_albedo,
+                     float *d_emission);
+
 /* Changes material parameters of a built scene in place (re-uploaded on next render). */
 int prt_scene_set_material_params(prt_scene *scene, uint32_t index, const float kd[3], float emission);
 

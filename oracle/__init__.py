@@ -65,7 +65,7 @@ def lib() -> C.CDLL:
             "oracle_tonemap": [C.c_uint32, FP, BP],
             "oracle_texture_table": [FP],
             "oracle_adjoint": [P, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Cfg), FP, FP, DP,
-                               DP, C.c_int],
+                               DP, C.c_double, C.c_int, C.c_int, C.c_int],
             "oracle_scene_set_material_params": [P, C.c_uint32, FP, C.c_float],
         }
         for name, args in sig.items():
@@ -147,13 +147,19 @@ class OracleScene:
                                 threads), "trace")
         return tri, t, u, v
 
-    def adjoint(self, cam: Camera, w: int, h: int, spp: int, c: Cfg, image, target, threads: int = 0):
+    def adjoint(self, cam: Camera, w: int, h: int, spp: int, c: Cfg, image, target, fp64: bool = False,
+                threads: int = 0, pathwise: bool = False, hess: float | None = None):
+        """hess None: the loss's own curvature 2/(w*h*3)."""
+        if hess is None:
+            hess = 2.0 / (w * h * 3)
         img = np.ascontiguousarray(image, np.float32)
         tgt = np.ascontiguousarray(target, np.float32)
         da = np.zeros((self.num_materials, 3), np.float64)
         de = np.zeros(self.num_materials, np.float64)
         _chk(lib().oracle_adjoint(self._h, C.byref(cam), w, h, spp, C.byref(c), _f(img), _f(tgt),
-                                  da.ctypes.data_as(DP), de.ctypes.data_as(DP), threads), "adjoint")
+                                  da.ctypes.data_as(DP), de.ctypes.data_as(DP), float(hess), int(pathwise),
+                                  int(fp64), threads),
+             "adjoint")
         return da, de
 
     def set_material_params(self, index: int, kd, emission: float) -> None:

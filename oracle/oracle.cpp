@@ -529,24 +529,33 @@ struct Interaction {
     uint32_t hitTriIdx = 0;
 };
 
-// One vertex record for the adjoint replay.
-template <class T>
-struct Tape {
-    bool on = false;
-};
-
-template <class T>
+// Path-replay adjoint state of one sample (fp64 whatever the path arithmetic).
 struct AdjointSink {
-    // Per-channel adjoint of this sample's radiance and running remainder.
-    V3<T> delta{0, 0, 0};
-    V3<T> rem{0, 0, 0};
-    double* dkd = nullptr;  // M*3
-    double* de = nullptr;   // M
+    bool pathwise = false;      // every roulette step as if p >= 1 (no score terms)
+    double halfHess = 0;        // (d2 Loss / dI2) / (2 spp^2)
+    V3<double> delta{0, 0, 0};  // dLoss / dL of this sample, per channel
+    V3<double> rem{0, 0, 0};    // radiance still to be collected on the replayed path
+    V3<double> L{0, 0, 0};      // the sample's radiance (primal pass)
+    uint32_t prevTies = 0;      // channels at the maximum at the previous roulette step (0: none)
+    std::vector<uint32_t> kdVerts;  // materials of the untextured diffuse vertices so far
+    double* dkd = nullptr;      // M*3
+    double* de = nullptr;       // M
 };
+template <class T>
+inline V3<double> d3(const V3<T>& v) { return {double(v.x), double(v.y), double(v.z)}; }
+
+// Weights of the channels in d log max(mask): 1 for a unique maximum, 1/2 for each
+// of several equal maxima (the mean of the one-sided derivatives of max, as a
+// central difference measures it).
+inline V3<double> rrWeight(uint32_t ties) {
+    const int n = int((ties & 1u) != 0) + int((ties & 2u) != 0) + int((ties & 4u) != 0);
+    const double w = n == 1 ? 1.0 : 0.5;
+    return {(ties & 1u) ? w : 0.0, (ties & 2u) ? w : 0.0, (ties & 4u) ? w : 0.0};
+}
 
 template <class T, bool kAdjoint>
 V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cfg& cfg, int x, int y,
-                  uint32_t sample, Counters& cnt, AdjointSink<T>* adj) {
+                  uint32_t sample, Counters& cnt, AdjointSink* adj) {
     Sampler smp;
     smp.key[0] = cfg.seed;
     smp.key[1] = cfg.view_id;
@@ -577,15 +586,15 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
         const Tri& tri = sc.tris[h.idx];
         const uint32_t m = tri.mat;
         const Mat& mat = sc.mats[m];
+        bool kdVertex = false;
         if (b == 0 || previousBsdf == PRT_SPECULAR || previousBsdf == PRT_REFRACTIVE) {
             V3<T> e = mask * load3<T>(mat.ka) * invLA;
             color = color + e;
             if (kAdjoint) {
                 // ka = (Ka_mtl * 3) * A_L and Ka_mtl = e_m (1,1,1): d/de_m = mask * 3 A_L * invLA.
-                T g = T(kLightsGain) * T(sc.lightsArea);
-                V3<T> de = mask * g * invLA;
-                adj->de[m] += double(dot(adj->delta, de));
-                adj->rem = adj->rem - e;
+                const double g = double(kLightsGain) * double(sc.lightsArea) * double(invLA);
+                adj->de[m] += dot(adj->delta, d3(mask) * g);
+                adj->rem = adj->rem - d3(e);
             }
         }
         it.normal = triNormal(tri, it.u, it.v);
@@ -667,13 +676,15 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
                 kd = texSample<T>(sc.texs[size_t(mat.tex)], tu, tv);
             }
             V3<T> ds = kd * cw;
+            kdVertex = !textured;
             if (kAdjoint && !textured) {
+                adj->kdVerts.push_back(m);
                 // Everything still to be collected on this path carries kd_c once.
-                const T kdc[3] = {T(mat.kd[0]), T(mat.kd[1]), T(mat.kd[2])};
-                const T rc[3] = {adj->rem.x, adj->rem.y, adj->rem.z};
-                const T dc[3] = {adj->delta.x, adj->delta.y, adj->delta.z};
+                const double kdc[3] = {mat.kd[0], mat.kd[1], mat.kd[2]};
+                const double rc[3] = {adj->rem.x, adj->rem.y, adj->rem.z};
+                const double dc[3] = {adj->delta.x, adj->delta.y, adj->delta.z};
                 for (int ch = 0; ch < 3; ++ch)
-                    if (kdc[ch] != T(0)) adj->dkd[m * 3 + ch] += double(dc[ch] * rc[ch] / kdc[ch]);
+                    if (kdc[ch] != 0.0) adj->dkd[m * 3 + ch] += dc[ch] * rc[ch] / kdc[ch];
             }
             mask = mask * ds / it.pdf;
 
@@ -703,15 +714,14 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
                     T wf = sa / (d2 * T(sc.lightsArea)) * cosT;
                     direct = direct + lightColor * wf;
                     if (kAdjoint) {
-                        T g = T(kLightsGain) * T(sc.lightsArea) * wf;
-                        V3<T> dl = mask * g;
-                        adj->de[L.mat] += double(dot(adj->delta, dl));
+                        const double g = double(kLightsGain) * double(sc.lightsArea) * double(wf);
+                        adj->de[L.mat] += dot(adj->delta, d3(mask) * g);
                     }
                 }
             }
             V3<T> nee = mask * direct;
             color = color + nee;
-            if (kAdjoint) adj->rem = adj->rem - nee;
+            if (kAdjoint) adj->rem = adj->rem - d3(nee);
         }
         if (currentBsdf == PRT_SPECULAR) {
             // sampleSpecularBSDF / reflect (renderer.cu:404-412)
@@ -725,10 +735,40 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
         ray.d = it.inputDirection;
         ray.tMin = T(kEps);
         ray.tMax = T(FLT_MAX);
+        T p = T(0);
+        uint32_t ties = 0;
         if (b >= cfg.rr_start) {
-            T p = maxComponent(mask);
+            p = maxComponent(mask);
+            ties = (mask.x == p ? 1u : 0u) | (mask.y == p ? 2u : 0u) | (mask.z == p ? 4u : 0u);
             if (T(smp.draw(blockB(b), 2)) > p) break;
             mask = mask / p;
+        }
+        if (kAdjoint && b >= cfg.rr_start) {
+            // Survived roulette step j. R_j = L - C_j (collected after the step)
+            // carries 1/p_j; with S_k[c] = sum over diffuse vertices v <= k of
+            // d log kd_{m_v,c}, d log p_j = w_j . S_j - w_{j-1} . S_{j-1}, and the step
+            // adds -kappa_j d log p_j: kappa_j = delta . R_j where p_j >= 1 (no random
+            // decision: mask /= p_j is a deterministic rescale); where p_j < 1 the
+            // score term of the survival probability, weighted by the loss change of
+            // the decision (delta . R_j - halfHess |R_j|^2), leaves
+            // kappa_j = halfHess |R_j|^2.
+            const V3<double> R = adj->L - d3(color);
+            const double kappa =
+                (p >= T(1) || adj->pathwise) ? dot(adj->delta, R) : adj->halfHess * dot(R, R);
+            const size_t before = adj->kdVerts.size() - (kdVertex ? 1 : 0);
+            if (kappa != 0.0) {
+                const V3<double> wj = rrWeight(ties), wp = rrWeight(adj->prevTies);
+                for (size_t i = 0; i < adj->kdVerts.size(); ++i) {
+                    const uint32_t mv = adj->kdVerts[i];
+                    const V3<double> cf = i < before ? wp - wj : wj * -1.0;
+                    const double c3[3] = {cf.x, cf.y, cf.z};
+                    for (int ch = 0; ch < 3; ++ch) {
+                        const double kdc = sc.mats[mv].kd[ch];
+                        if (c3[ch] != 0.0 && kdc != 0.0) adj->dkd[mv * 3 + ch] += kappa * c3[ch] / kdc;
+                    }
+                }
+            }
+            adj->prevTies = ties;
         }
     }
     return color;
@@ -776,6 +816,42 @@ inline uint8_t gammaByte(uint8_t b) {
     float invGamma = 1.f / 2.2f;
     float f = std::pow(float(b) * 1.f / 255.f, invGamma);
     return uint8_t(f * 255.f);
+}
+
+template <class T>
+void adjointPixels(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h, uint32_t spp,
+                   const oracle_cfg* cfg, const float* image, const float* target, std::vector<double>& gk,
+                   std::vector<double>& ge, int nt, double hess, bool pathwise) {
+    const size_t M = s->mats.size();
+    const double norm = 2.0 / (double(w) * double(h) * 3.0 * double(spp));
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = omp_get_thread_num();
+        AdjointSink sink;
+        sink.pathwise = pathwise;
+        sink.halfHess = hess / (2.0 * double(spp) * double(spp));
+        sink.dkd = &gk[size_t(tid) * M * 3];
+        sink.de = &ge[size_t(tid) * M];
+        Counters c;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t y = 0; y < int64_t(h); ++y)
+            for (uint32_t x = 0; x < w; ++x) {
+                size_t idx = (size_t(y) * w + x) * 3;
+                V3<double> delta{norm * (double(image[idx]) - double(target[idx])),
+                                 norm * (double(image[idx + 1]) - double(target[idx + 1])),
+                                 norm * (double(image[idx + 2]) - double(target[idx + 2]))};
+                for (uint32_t k = 0; k < spp; ++k) {
+                    // primal pass, then the replay with the same random numbers
+                    V3<T> L = samplePixel<T, false>(*s, *cam, *cfg, int(x), int(y), k, c, nullptr);
+                    sink.delta = delta;
+                    sink.L = d3(L);
+                    sink.rem = d3(L);
+                    sink.prevTies = 0;
+                    sink.kdVerts.clear();
+                    samplePixel<T, true>(*s, *cam, *cfg, int(x), int(y), k, c, &sink);
+                }
+            }
+    }
 }
 
 }  // namespace
@@ -1106,36 +1182,15 @@ int oracle_texture_table(float out[256]) {
 
 int oracle_adjoint(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h, uint32_t spp,
                    const oracle_cfg* cfg, const float* image, const float* target, double* d_albedo,
-                   double* d_emission, int threads) {
+                   double* d_emission, double hess, int pathwise, int fp64, int threads) {
     if (!s || !cam || !cfg || !image || !target || !d_albedo || !d_emission || spp == 0) return PRT_ERR_INVALID_ARG;
     if (s->lights.empty()) return PRT_ERR_NO_EMITTERS;
     if (int64_t(w) != cam->xpixels || int64_t(h) != cam->ypixels) return PRT_ERR_INVALID_ARG;
     const size_t M = s->mats.size();
     int nt = threads > 0 ? threads : omp_get_max_threads();
     std::vector<double> gk(size_t(nt) * M * 3, 0.0), ge(size_t(nt) * M, 0.0);
-    const double norm = 2.0 / (double(w) * double(h) * 3.0 * double(spp));
-#pragma omp parallel num_threads(nt)
-    {
-        int tid = omp_get_thread_num();
-        AdjointSink<double> sink;
-        sink.dkd = &gk[size_t(tid) * M * 3];
-        sink.de = &ge[size_t(tid) * M];
-        Counters c;
-#pragma omp for schedule(dynamic, 1)
-        for (int64_t y = 0; y < int64_t(h); ++y)
-            for (uint32_t x = 0; x < w; ++x) {
-                size_t idx = (size_t(y) * w + x) * 3;
-                V3<double> delta{norm * (double(image[idx]) - double(target[idx])),
-                                 norm * (double(image[idx + 1]) - double(target[idx + 1])),
-                                 norm * (double(image[idx + 2]) - double(target[idx + 2]))};
-                for (uint32_t k = 0; k < spp; ++k) {
-                    V3<double> L = samplePixel<double, false>(*s, *cam, *cfg, int(x), int(y), k, c, nullptr);
-                    sink.delta = delta;
-                    sink.rem = L;
-                    samplePixel<double, true>(*s, *cam, *cfg, int(x), int(y), k, c, &sink);
-                }
-            }
-    }
+    if (fp64) adjointPixels<double>(s, cam, w, h, spp, cfg, image, target, gk, ge, nt, hess, pathwise != 0);
+    else adjointPixels<float>(s, cam, w, h, spp, cfg, image, target, gk, ge, nt, hess, pathwise != 0);
     for (size_t i = 0; i < M * 3; ++i) {
         double a = 0;
         for (int t = 0; t < nt; ++t) a += gk[size_t(t) * M * 3 + i];

@@ -72,12 +72,20 @@ int oracle_camera_ray(const prt_camera *cam, int32_t x, int32_t y, float u0, flo
 int oracle_tonemap(uint32_t n, const float *rgb, uint8_t *rgba);
 int oracle_texture_table(float out[256]);
 
-/* Path-replay adjoint, fp32 or fp64: gradients of sum (I - T)^2 / (W*H*3) with respect to
- * every material's kd (RGB) and emission scale. image = mean radiance of the forward pass
- * over the same spp and seed. d_albedo: M*3, d_emission: M (overwritten). */
+/* Path-replay adjoint: gradients of sum (I - T)^2 / (W*H*3) with respect to every
+ * material's kd (RGB) and emission scale, accumulated in fp64. The replayed paths use
+ * fp32 arithmetic (fp64 = 0, the same paths the HIP kernels trace) or fp64 (fp64 = 1).
+ * image = mean radiance of the forward pass over the same spp and seed.
+ * hess = d2Loss/dI2 per element weighting the roulette decisions (2/(W*H*3) for this
+ * loss; 0 for the first-order estimate of the gradient of the loss of E[I]).
+ * pathwise = 0: derivative of the expected loss (the HIP kernels' estimator);
+ * pathwise = 1: derivative of the loss for these fixed random numbers, with every
+ * roulette decision held (1/p differentiated at every step) -- what a finite
+ * difference with a step too small to flip a decision measures.
+ * d_albedo: M*3, d_emission: M (overwritten). */
 int oracle_adjoint(const oracle_scene *s, const prt_camera *cam, uint32_t w, uint32_t h, uint32_t spp,
                    const oracle_cfg *cfg, const float *image, const float *target, double *d_albedo,
-                   double *d_emission, int threads);
+                   double *d_emission, double hess, int pathwise, int fp64, int threads);
 /* Replaces kd and emission scale of material i (ka = e * (1,1,1) in MTL units). */
 int oracle_scene_set_material_params(oracle_scene *s, uint32_t index, const float kd[3], float emission);
 

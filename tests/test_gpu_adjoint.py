@@ -1,0 +1,157 @@
+"""Path-replay adjoint on the GPU.
+
+Checks: (1) equality with the oracle's replay of the same fp32 paths (only the
+fp64 summation order differs); (2) central finite differences (h = 1e-3, common
+random numbers) of the GPU forward renders: without Russian roulette to 1e-3
+(relative L2 of the whole gradient vector); on the 512x512 64-sample configuration
+with roulette, the mean over seeds (tests/adjoint_fd.py);
+(3) first-order gradients (no roulette curvature term) against the oracle's fp64
+paths at 4x the samples, the loss adjoint taken from 256-sample images
+(statistical tolerance 10%);
+(4) the torch.autograd harness returns the adjoint's gradients.
+"""
+import numpy as np
+import pytest
+
+import cudart_amd as ca
+import oracle as O
+from cudart_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(w, h, spp, seed=1, max_depth=None, rr_start=3, render_seed=W.SEED):
+    # Scene, renderer and a target from perturbed parameters (another seed).
+    wl = W.cornell100k(w, h)
+    if max_depth is not None:
+        wl.max_depth = max_depth
+    scene = ca.Scene(wl.mesh)
+    mats = wl.mesh.materials
+    kd0 = np.array([list(m.kd) for m in mats], np.float32)
+    em0 = np.array([m.ka[0] for m in mats], np.float32)
+    rng = np.random.default_rng(seed)
+    kdT = np.clip(kd0 + rng.uniform(-0.1, 0.1, kd0.shape), 0.05, 0.95).astype(np.float32)
+    emT = (em0 + np.where(em0 > 0, rng.uniform(-0.1, 0.1, len(mats)), 0)).astype(np.float32)
+    r = ca.ParallelRenderer(scene, w, h, spp, camera=wl.camera, max_depth=wl.max_depth, rr_start=rr_start,
+                            seed=render_seed, integrator=ca.INTEGRATOR_WAVEFRONT)
+
+    def render(kd, em):
+        for i in range(len(mats)):
+            scene.set_material_params(i, kd[i], float(em[i]))
+        r.reset()
+        r.render(spp)
+        return r.radiance()
+
+    # The target comes from other random numbers than the image (as a photograph
+    # would): a target correlated with the image's own roulette decisions would bias
+    # the comparison of the boundary terms of those decisions.
+    rt = ca.ParallelRenderer(scene, w, h, spp, camera=wl.camera, max_depth=wl.max_depth, rr_start=rr_start,
+                             seed=render_seed ^ 0x7A5C3E1, integrator=ca.INTEGRATOR_WAVEFRONT)
+    for i in range(len(mats)):
+        scene.set_material_params(i, kdT[i], float(emT[i]))
+    rt.render(spp)
+    target = rt.radiance()
+    rt.close()
+    image = render(kd0, em0)
+    return wl, scene, r, render, kd0, em0, target, image
+
+
+def test_adjoint_equals_oracle_replay(gpu):
+    wl, scene, r, render, kd0, em0, target, image = _setup(64, 64, 4)
+    d_alb, d_em, loss = r.adjoint(target, 4, image=image)
+    orc = O.OracleScene(wl.mesh.desc())
+    oa, oe = orc.adjoint(wl.camera, 64, 64, 4, O.cfg(max_depth=wl.max_depth, seed=W.SEED), image, target)
+    g = np.concatenate([d_alb.ravel(), d_em]).astype(np.float64)
+    og = np.concatenate([oa.ravel(), oe])
+    scale = np.abs(og).max()
+    assert scale > 0
+    np.testing.assert_allclose(g, og, rtol=1e-5, atol=1e-7 * scale)
+    assert loss == pytest.approx(float(np.mean((image.astype(np.float64) - target) ** 2)), rel=1e-6)
+
+
+def _fd_gradient(render, kd0, em0, loss, h=1e-3):
+    M = len(em0)
+    fd_alb = np.zeros((M, 3))
+    fd_em = np.zeros(M)
+    for m in range(M):
+        for c in range(3):
+            kp, km = kd0.copy(), kd0.copy()
+            kp[m, c] += h
+            km[m, c] -= h
+            fd_alb[m, c] = (loss(render(kp, em0)) - loss(render(km, em0))) / (2 * h)
+        ep, en = em0.copy(), em0.copy()
+        ep[m] += h
+        en[m] -= h
+        fd_em[m] = (loss(render(kd0, ep)) - loss(render(kd0, en))) / (2 * h)
+    return np.concatenate([fd_alb.ravel(), fd_em])
+
+
+def test_adjoint_matches_central_differences_without_roulette(gpu):
+    # Without Russian roulette (rr_start beyond max_depth) the estimator is a smooth
+    # function of albedo and emission for fixed random numbers, so central differences
+    # of the same renders must reproduce the adjoint closely.
+    spp = 16
+    wl, scene, r, render, kd0, em0, target, image = _setup(128, 128, spp, max_depth=4, rr_start=8)
+    d_alb, d_em, _ = r.adjoint(target, spp, image=image)
+    loss = lambda img: float(np.mean((img.astype(np.float64) - target) ** 2))  # noqa: E731
+    fd = _fd_gradient(render, kd0, em0, loss)
+    g = np.concatenate([d_alb.ravel(), d_em]).astype(np.float64)
+    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+    print(f"no roulette: relative L2 error vs central FD {rel:.2e}")
+    assert rel <= 1e-3
+    big = np.abs(fd) >= 0.1 * np.abs(fd).max()
+    np.testing.assert_allclose(g[big], fd[big], rtol=5e-3)
+
+
+def test_adjoint_matches_central_differences_c4(gpu):
+    # 512x512, 64 samples, depth 6 with Russian roulette from bounce 3: the mean over
+    # seeds of the adjoint against the mean of central differences (h = 1e-3). The
+    # roulette flips inside +-h make each seed's differences noisy (2-4% relative L2
+    # per seed); six seeds bring the noise of the mean near 1.3%, and the standalone
+    # 48-seed run of tests/adjoint_fd.py measures 0.53%. Without the roulette curvature
+    # term the adjoint is biased by about 5.5%, which this test must resolve.
+    from tests.adjoint_fd import compare, seed_list
+    errs, g, fd, g1 = compare(seed_list(6))
+    first_order = np.linalg.norm(g1 - fd) / np.linalg.norm(fd)
+    print(f"C4: relative L2 error of the mean over 6 seeds {errs[-1]:.4f} (first-order {first_order:.4f})")
+    assert errs[-1] <= 0.025
+    assert first_order >= 2 * errs[-1]
+    big = np.abs(fd) >= 0.2 * np.abs(fd).max()
+    np.testing.assert_allclose(g[big], fd[big], rtol=0.03)
+
+
+def test_adjoint_close_to_high_spp_fp64_reference(gpu):
+    # Loss adjoint from nearly noise-free images (256 spp), so both sides estimate the
+    # same quantity sum_p dLoss/dI_p * E[dI_p/dtheta]: the GPU with 64 samples of the
+    # derivative, the oracle with 256 samples on fp64 paths.
+    # First-order estimates (hess = 0) of the gradient of the loss of the expected image.
+    wl, scene, r, render, kd0, em0, target, image = _setup(48, 48, 256)
+    spp = 64
+    grad = (2.0 / image.size) * (image.astype(np.float64) - target)
+    d_alb, d_em = r.adjoint_grad(grad.astype(np.float32), spp, hess=0.0)
+    orc = O.OracleScene(wl.mesh.desc())
+    oa, oe = orc.adjoint(wl.camera, 48, 48, 256, O.cfg(max_depth=wl.max_depth, seed=W.SEED), image, target,
+                         fp64=True, hess=0.0)
+    g = np.concatenate([d_alb.ravel(), d_em]).astype(np.float64)
+    og = np.concatenate([oa.ravel(), oe])
+    rel = np.linalg.norm(g - og) / np.linalg.norm(og)
+    print(f"relative L2 difference vs 256-spp fp64 reference: {rel:.4f}")
+    assert rel < 0.10  # Monte Carlo noise of the two estimates (about 0.05 expected)
+    big = np.abs(og) >= 0.2 * np.abs(og).max()
+    assert (np.sign(g[big]) == np.sign(og[big])).all()
+
+
+def test_torch_autograd_harness(gpu):
+    torch = pytest.importorskip("torch")
+    from cudart_amd.torch_render import render as trender
+    spp = 4
+    wl, scene, r, render, kd0, em0, target, image = _setup(48, 40, spp)
+    albedo = torch.tensor(kd0, requires_grad=True)
+    emission = torch.tensor(em0, requires_grad=True)
+    img = trender(albedo, emission, r, spp, 2.0 / image.size)
+    assert np.array_equal(img.detach().numpy(), image)
+    loss = ((img - torch.from_numpy(target)) ** 2).mean()
+    loss.backward()
+    d_alb, d_em, _ = r.adjoint(target, spp, image=image)
+    np.testing.assert_allclose(albedo.grad.numpy(), d_alb, rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(emission.grad.numpy(), d_em, rtol=1e-4, atol=1e-7)
