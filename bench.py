@@ -1,30 +1,45 @@
-"""Benchmark: Mrays/s and ms/frame of the path tracer on the 1M-triangle scene.
+"""Benchmark: Mrays/s and ms/frame of the path tracer on the 1M-triangle scene,
+the megakernel beside it, and the adjoint's grad-samples/s.
 
 Workload (one "step" = one frame): 200 displaced icospheres + ground + 2 emissive
 quads (1,024,006 triangles), 1920x1080, 128 spp, depth 8 with Russian roulette
-from bounce 3, seed 0x5EED. The scene is generated and its BVH built and uploaded
-before timing; the timed region renders K whole frames (accumulation buffer
-cleared per frame), bracketed by device synchronisation.
+from bounce 3, seed 0x5EED. The scene is generated, its BVH built and uploaded
+before timing; the timed region renders K whole frames with the persistent
+wavefront integrator (one HIP graph per chunk of paths; accumulation cleared per
+frame), bracketed by device synchronisation. `value` = rays traced per second.
 
-Rays = primary + extension + shadow rays traced, counted by the kernel on an
-instrumented warm-up frame (the counts of a frame are deterministic for a seed).
+Rays = primary + extension + shadow rays, counted by the kernels on an
+instrumented frame (the counts of a frame are deterministic for a seed).
 
-Roofline: the dominant kernel is the megakernel launch (spp_per_launch samples of
-every pixel). Its algorithmic bytes per launch are
-    64 B x child-pair nodes fetched + 48 B x triangle records fetched + 64 B x
-    shading records fetched,
-counted by the kernel; achieved = bytes / (average launch time measured with HIP
-events on the renderer's stream), peak = 8 TB/s HBM3E.
+Megakernel leg: the same frame with the one-kernel-per-launch integrator; its
+image must equal the wavefront's bit for bit (so the RMSE against any reference
+is the same), and the ratio of the two rates is reported.
+
+Roofline of the dominant kernel (wf_extend, closest-hit traversal of every primary
+and extension ray): algorithmic bytes per launch
+    64 B x child-pair nodes + 48 B x triangle records + 64 B x shading records
+of the closest-hit rays (device counters of the instrumented frame, shadow-ray
+share removed), divided by the average extend launch time measured with HIP events
+around every extend launch on the renderer's stream (a frame rendered with the
+chunk's kernels launched directly instead of through the graph); peak = 8 TB/s.
+`traffic` = measured HBM bytes per extend launch from rocprofv3 FETCH_SIZE and
+WRITE_SIZE passes scaled by the same counters on a plain copy kernel (read from
+the committed summary under profiles/, null when absent).
+
+Adjoint leg: path-replay gradients of the L2 image loss w.r.t. 16 materials'
+albedo and emission on the 100k-triangle Cornell-like scene at 512x512, 64
+samples replayed: grad-samples/s = pixels x samples / adjoint device time.
 
 CPU baseline: the oracle (C++ restatement of the reference --cpu renderer, fp32,
-OpenMP) on rank 0 over a fixed sample of pixels of the same frame at 1 spp,
-reported in Mrays/s with the thread count.
+OpenMP) on a bounded sample of pixels of the same frame, in Mrays/s with the
+thread count.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -36,6 +51,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s + ms/frame @1080p 128spp 1M tris; adjoint grad-samples/s; CPU-relative"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "wf_extend_spheres1m_1080p_hbm_traffic.txt"
 
 
 def parse():
@@ -48,12 +64,30 @@ def parse():
     p.add_argument("--spp", type=int, default=128)
     p.add_argument("--depth", type=int, default=8)
     p.add_argument("--instances", type=int, default=200)
-    p.add_argument("--spp-per-launch", type=int, default=4)
-    p.add_argument("--integrator", choices=["megakernel", "wavefront"], default="megakernel")
+    p.add_argument("--spp-per-launch", type=int, default=4, help="megakernel samples per launch")
+    p.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
+    p.add_argument("--no-compare", action="store_true", help="skip the megakernel leg")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--threads", type=int, default=0)
+    p.add_argument("--frames-only", action="store_true",
+                   help="only the timed frames of the headline integrator (for profilers)")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
     return p.parse_args()
+
+
+def copy_probe():
+    import torch
+
+    n = 1 << 28  # 2^28 float32 = 1 GiB read + 1 GiB written per copy
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
 
 
 def cpu_baseline(wl, target_s: float, threads: int):
@@ -78,20 +112,80 @@ def cpu_baseline(wl, target_s: float, threads: int):
     _, cnt = orc.render_pixels(wl.camera, wl.width, wl.height, pix, 0, spp, c, threads)
     dt = time.perf_counter() - t0
     rays = int(cnt.sum())
-    used = threads if threads > 0 else (os.cpu_count() or 1)
     frame_ms = dt * (npx / n) * (wl.spp / spp) * 1e3
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": used, "kind": "port",
+    return {"value": round(rays / dt / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{n} random pixels x samples 0..{spp - 1} of the same {wl.width}x{wl.height} frame: "
-                      f"{rays} rays in {dt:.1f} s on {used} OpenMP threads",
+                      f"{rays} rays in {dt:.1f} s on {threads} OpenMP threads",
             "ms_per_frame_extrapolated": round(frame_ms, 1)}
+
+
+def measured_traffic():
+    """HBM bytes per wf_extend launch from the committed rocprofv3 summary, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def timed_frames(r, spp, warmup, steps):
+    for _ in range(warmup):
+        r.reset()
+        r.render(spp)
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.reset()
+        r.render(spp)
+        dev.append(r.stats().last_render_ms)
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(dev))
+
+
+def adjoint_leg():
+    import cudart_amd as ca
+    from cudart_amd import workloads as W
+
+    wl = W.cornell100k(512, 512, 64)
+    scene = ca.Scene(wl.mesh)
+    M = scene.num_materials
+    kd0 = np.array([list(m.kd) for m in wl.mesh.materials], np.float32)
+    em0 = np.array([m.ka[0] for m in wl.mesh.materials], np.float32)
+    rng = np.random.default_rng(1)
+    kdT = np.clip(kd0 + rng.uniform(-0.1, 0.1, kd0.shape), 0.05, 0.95).astype(np.float32)
+    kw = dict(camera=wl.camera, max_depth=wl.max_depth, integrator=ca.INTEGRATOR_WAVEFRONT)
+    rt = ca.ParallelRenderer(scene, 512, 512, 64, seed=W.SEED ^ 0x7A5C3E1, **kw)
+    for i in range(M):
+        scene.set_material_params(i, kdT[i], float(em0[i]))
+    rt.render(64)
+    target = rt.radiance()
+    rt.close()
+    for i in range(M):
+        scene.set_material_params(i, kd0[i], float(em0[i]))
+    r = ca.ParallelRenderer(scene, 512, 512, 64, seed=W.SEED, **kw)
+    r.render(64)
+    image = r.radiance()
+    r.adjoint(target, 64, image=image)  # warm-up
+    times = []
+    for _ in range(3):
+        r.adjoint(target, 64, image=image)
+        times.append(r.stats().last_render_ms)
+    ms = float(np.median(times))
+    r.close()
+    return {"value": round(512 * 512 * 64 / (ms * 1e-3) / 1e6, 2), "unit": "Mgrad-samples/s",
+            "ms": round(ms, 3), "config": {"workload": "cornell100k: 100k-triangle Cornell-like scene, 16 materials",
+                                           "width": 512, "height": 512, "spp_replayed": 64, "max_depth": 6,
+                                           "parameters": 4 * M}}
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards by sample range, "
-              f"see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
+              f"(the path would shard by sample range and tile across GPUs)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
     import cudart_amd as ca
     from cudart_amd import workloads as W
 
@@ -100,37 +194,55 @@ def main():
     scene = ca.Scene(wl.mesh)
     build_s = time.perf_counter() - t0
     info = scene.bvh_info()
-    integ = ca.INTEGRATOR_WAVEFRONT if a.integrator == "wavefront" else ca.INTEGRATOR_MEGAKERNEL
+    wave = a.integrator == "wavefront"
+    integ = ca.INTEGRATOR_WAVEFRONT if wave else ca.INTEGRATOR_MEGAKERNEL
     per = max(1, a.spp_per_launch)
-    launches = (a.spp + per - 1) // per
+    base = dict(camera=wl.camera, max_depth=a.depth, seed=W.SEED, spp_per_launch=per)
 
-    # Instrumented warm-up frame: ray / node / triangle counts of one frame.
-    rc = ca.ParallelRenderer(scene, a.width, a.height, a.spp, camera=wl.camera, max_depth=a.depth, seed=W.SEED,
-                             integrator=integ, spp_per_launch=per, count_rays=1)
+    r = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ, **base)
+    if a.frames_only:
+        wall, dev_ms = timed_frames(r, a.spp, a.warmup, a.steps)
+        print(json.dumps({"frames": a.steps, "ms_per_frame": round(dev_ms, 3)}))
+        return
+
+    # Instrumented frame: ray / node / triangle counts of one frame.
+    rc = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ, count_rays=1, **base)
     rc.render(a.spp)
     st = rc.stats()
-    counted_ms = st.last_render_ms
-    rays = st.primary_rays + st.extension_rays + st.shadow_rays
-    alg_bytes = 64 * st.nodes_visited + 48 * st.tris_tested + 64 * st.hits
     rc.close()
+    rays = st.primary_rays + st.extension_rays + st.shadow_rays
 
-    r = ca.ParallelRenderer(scene, a.width, a.height, a.spp, camera=wl.camera, max_depth=a.depth, seed=W.SEED,
-                            integrator=integ, spp_per_launch=per)
-    for _ in range(a.warmup):
-        r.reset()
-        r.render(a.spp)
-    frame_ms = []
-    t_start = time.perf_counter()
-    for _ in range(a.steps):
-        r.reset()
-        r.render(a.spp)
-        frame_ms.append(r.stats().last_render_ms)
-    wall = time.perf_counter() - t_start
-    dev_ms = float(np.mean(frame_ms))
+    wall, dev_ms = timed_frames(r, a.spp, a.warmup, a.steps)
     ms_per_step = wall * 1e3 / a.steps
     value = rays * a.steps / wall / 1e6
-    launch_ms = dev_ms / launches
-    achieved = alg_bytes / launches / (launch_ms * 1e-3) / 1e9
+    image = r.radiance()
+    r.close()
+
+    # Roofline of the dominant kernel.
+    if wave:
+        rp = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ, profile_kernels=1, **base)
+        rp.render(a.spp)
+        sp = rp.stats()
+        rp.close()
+        launches = sp.kernel_launches[0]
+        launch_ms = sp.kernel_ms[0] / launches
+        ext_bytes = 64 * (st.nodes_visited - st.anyhit_nodes) + 48 * (st.tris_tested - st.anyhit_tris) + 64 * st.hits
+        bytes_per_launch = ext_bytes / launches
+        kernel = "wf_extend"
+        kernel_times = {"extend": round(sp.kernel_ms[0], 2), "shade": round(sp.kernel_ms[1], 2),
+                        "shadow": round(sp.kernel_ms[2], 2), "launches_each": launches,
+                        "profiled_frame_ms": round(sp.last_render_ms, 2)}
+        per_ray = {"nodes": round((st.nodes_visited - st.anyhit_nodes) / (st.primary_rays + st.extension_rays), 2),
+                   "tris": round((st.tris_tested - st.anyhit_tris) / (st.primary_rays + st.extension_rays), 2)}
+    else:
+        launches = (a.spp + per - 1) // per
+        launch_ms = dev_ms / launches
+        bytes_per_launch = (64 * st.nodes_visited + 48 * st.tris_tested + 64 * st.hits) / launches
+        kernel = "megakernel"
+        kernel_times = None
+        per_ray = {"nodes": round(st.nodes_visited / rays, 2), "tris": round(st.tris_tested / rays, 2)}
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if wave else None
 
     out = {
         "metric": METRIC,
@@ -147,20 +259,31 @@ def main():
         "data": "synthetic",
         "config": {"workload": "spheres1m: 200 displaced L4 icospheres + ground + 2 emissive quads, seed 0x5EED",
                    "triangles": scene.num_triangles, "width": a.width, "height": a.height, "spp": a.spp,
-                   "max_depth": a.depth, "rr_start": 3, "integrator": a.integrator, "spp_per_launch": per},
+                   "max_depth": a.depth, "rr_start": 3, "integrator": a.integrator},
         "ms_per_frame": round(dev_ms, 3),
         "rays_per_frame": int(rays),
         "rays_by_kind": {"primary": int(st.primary_rays), "extension": int(st.extension_rays),
                          "shadow": int(st.shadow_rays)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "megakernel" if integ == ca.INTEGRATOR_MEGAKERNEL else "wavefront",
-                     "launch_ms": round(launch_ms, 4), "bytes_per_launch": int(alg_bytes // launches),
-                     "per_ray": {"nodes": round(st.nodes_visited / rays, 2), "tris": round(st.tris_tested / rays, 2)}},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+                     "launch_ms": round(launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+                     "per_ray": per_ray, "kernel_ms_per_frame": kernel_times},
         "bvh": {"nodes": info.num_nodes, "max_depth": info.max_depth, "sah_cost": round(info.sah_cost, 2),
                 "build_s": round(build_s, 2)},
-        "counted_frame_ms": round(counted_ms, 3),
     }
+    if wave and not a.no_compare:
+        rm = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=ca.INTEGRATOR_MEGAKERNEL, **base)
+        m_wall, m_dev = timed_frames(rm, a.spp, 1, max(1, min(2, a.steps)))
+        m_steps = max(1, min(2, a.steps))
+        m_image = rm.radiance()
+        rm.close()
+        m_value = rays * m_steps / m_wall / 1e6
+        out["megakernel"] = {"value": round(m_value, 2), "unit": "Mrays/s", "ms_per_frame": round(m_dev, 3),
+                             "spp_per_launch": per, "steps": m_steps,
+                             "image_equal_to_wavefront": bool(np.array_equal(image, m_image))}
+        out["wavefront_over_megakernel"] = round(value / m_value, 3)
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds, a.threads or min(16, os.cpu_count() or 1))
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

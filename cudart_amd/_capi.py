@@ -47,13 +47,15 @@ class SceneDesc(C.Structure):
 class RenderConfig(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32), ("view_id", C.c_uint32),
                 ("device", C.c_int32), ("integrator", C.c_uint32), ("spp_per_launch", C.c_uint32),
-                ("count_rays", C.c_uint32), ("wavefront_chunk", C.c_uint32), ("wavefront_queue", C.c_uint32)]
+                ("count_rays", C.c_uint32), ("wavefront_chunk", C.c_uint32), ("wavefront_queue", C.c_uint32),
+                ("profile_kernels", C.c_uint32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64), ("hits", C.c_uint64),
-                ("samples", C.c_uint64), ("last_render_ms", C.c_double)]
+                ("samples", C.c_uint64), ("last_render_ms", C.c_double), ("anyhit_nodes", C.c_uint64),
+                ("anyhit_tris", C.c_uint64), ("kernel_ms", C.c_double * 3), ("kernel_launches", C.c_uint32 * 3)]
 
 
 class BvhInfo(C.Structure):

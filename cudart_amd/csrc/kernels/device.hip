@@ -374,6 +374,12 @@ extern "C" int prt_get_stats(prt_renderer* r, prt_stats* st) {
     st->hits = c[kCntHits];
     st->samples = r->samples_total;
     st->last_render_ms = r->last_ms;
+    st->anyhit_nodes = c[kCntAnyNodes];
+    st->anyhit_tris = c[kCntAnyTris];
+    for (int k = 0; k < 3; ++k) {
+        st->kernel_ms[k] = r->kernel_ms[k];
+        st->kernel_launches[k] = r->kernel_launches[k];
+    }
     return PRT_OK;
 }
 

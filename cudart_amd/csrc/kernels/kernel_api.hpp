@@ -45,7 +45,18 @@ struct PathArgs {
 };
 
 // Counter slots (unsigned long long each).
-enum CounterSlot { kCntPrimary = 0, kCntExtension, kCntShadow, kCntNodes, kCntTris, kCntHits, kCntSamples, kCntNum };
+enum CounterSlot {
+    kCntPrimary = 0,
+    kCntExtension,
+    kCntShadow,
+    kCntNodes,
+    kCntTris,
+    kCntHits,
+    kCntSamples,
+    kCntAnyNodes,  // wavefront shadow kernel's share of kCntNodes
+    kCntAnyTris,   // and of kCntTris
+    kCntNum
+};
 
 struct MegaArgs {
     SceneArgs sc;

@@ -29,6 +29,8 @@ struct prt_renderer {
     uint32_t samples_done = 0;
     uint64_t samples_total = 0;
     double last_ms = 0.0;
+    double kernel_ms[3] = {0, 0, 0};  // profile_kernels: extend, shade, shadow of the last wavefront render
+    uint32_t kernel_launches[3] = {0, 0, 0};
     prt::WavefrontState* wf = nullptr;
 };
 

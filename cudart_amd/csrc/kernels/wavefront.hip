@@ -155,7 +155,9 @@ __device__ __forceinline__ SpreadFetch make_fetch(uint32_t* ctr, uint32_t n) {
     return SpreadFetch{ctr, wave % kSpread, (n + 63) / 64};
 }
 
-template <bool COUNT>
+// Adds the wave's counts to the counter slots; ANYHIT also books nodes and
+// triangles as the shadow (any-hit) kernel's share.
+template <bool ANYHIT>
 __device__ __forceinline__ void flush_stats(uint32_t* ctr, const Counts& c) {
     unsigned long long v[6] = {c.primary, c.extension, c.shadow, c.nodes, c.tris, c.hits};
     unsigned long long* out = reinterpret_cast<unsigned long long*>(ctr + kStats);
@@ -164,7 +166,11 @@ __device__ __forceinline__ void flush_stats(uint32_t* ctr, const Counts& c) {
         unsigned long long x = v[k];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(out + k, x);
+        if ((threadIdx.x & 63) == 0 && x) {
+            atomicAdd(out + k, x);
+            if (ANYHIT && k == kCntNodes) atomicAdd(out + kCntAnyNodes, x);
+            if (ANYHIT && k == kCntTris) atomicAdd(out + kCntAnyTris, x);
+        }
     }
 }
 
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
         });
         base = fetch.resolve(pending);
     }
-    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+    if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
 
 // Appends to two queues for a whole workgroup with one atomic per queue: lanes
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
             a.sh[si] = make_float4(ns.lv, __uint_as_float(more ? 0u : 1u), __uint_as_float(pj >> 24), 0.f);
         }
     }
-    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+    if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
 
 template <bool COUNT>
@@ -428,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
         });
         base = fetch.resolve(pending);
     }
-    if (COUNT) flush_stats<COUNT>(a.ctr, cnt);
+    if (COUNT) flush_stats<true>(a.ctr, cnt);
 }
 
 __global__ __launch_bounds__(256) void wf_resolve(WfArgs a) {
@@ -583,12 +589,81 @@ void free_state(WavefrontState& w) {
 struct RunStats {
     unsigned long long c[kCntNum];
     double ms;
+    double kernel_ms[3];  // profile: extend, shade, shadow
+    uint32_t launches[3];
 };
+
+// Launches one chunk's kernels on `stream`: setup, generate, then per iteration
+// extend, shade and shadow, then resolve and advance. With `ev` (4 per iteration)
+// the extend / shade / shadow launches are bracketed by events.
+template <bool COUNT>
+void launch_chunk(const WfArgs& a, hipStream_t stream, uint32_t max_depth, int pblocks, int gblocks, int sblocks,
+                  hipEvent_t* ev) {
+    hipLaunchKernelGGL(dev::wf_setup, dim3(1), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(dev::wf_generate, dim3(gblocks), dim3(256), 0, stream, a);
+    for (uint32_t it = 0; it < max_depth; ++it) {
+        if (ev) (void)hipEventRecord(ev[4 * it], stream);
+        hipLaunchKernelGGL(dev::wf_extend<COUNT>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        if (ev) (void)hipEventRecord(ev[4 * it + 1], stream);
+        hipLaunchKernelGGL(dev::wf_shade<COUNT>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
+        if (ev) (void)hipEventRecord(ev[4 * it + 2], stream);
+        hipLaunchKernelGGL(dev::wf_shadow<COUNT>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        if (ev) (void)hipEventRecord(ev[4 * it + 3], stream);
+    }
+    hipLaunchKernelGGL(dev::wf_resolve, dim3(gblocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(dev::wf_advance, dim3(1), dim3(64), 0, stream, a);
+}
+
+// Profile mode: every chunk launched directly, each extend / shade / shadow launch
+// timed with events; per-kernel device time summed into rs.
+int run_profiled(const WfArgs& a, hipStream_t stream, size_t nchunks, uint32_t max_depth, int pblocks, int gblocks,
+                 int sblocks, bool count, RunStats* rs) {
+    std::vector<hipEvent_t> ev(4 * max_depth, nullptr);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PRT_OK;
+    double total = 0.0;
+    double kms[3] = {0, 0, 0};
+    hipError_t e = hipSuccess;
+    for (auto& x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (size_t c = 0; c < nchunks && e == hipSuccess; ++c) {
+        e = hipEventRecord(e0, stream);
+        if (e != hipSuccess) break;
+        if (count) launch_chunk<true>(a, stream, max_depth, pblocks, gblocks, sblocks, ev.data());
+        else launch_chunk<false>(a, stream, max_depth, pblocks, gblocks, sblocks, ev.data());
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(e1, stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        total += ms;
+        for (uint32_t it = 0; it < max_depth && e == hipSuccess; ++it)
+            for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+                e = hipEventElapsedTime(&ms, ev[4 * it + k], ev[4 * it + k + 1]);
+                kms[k] += ms;
+            }
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "profiled wavefront launch");
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc == PRT_OK && rs) {
+        rs->ms = total;
+        for (int k = 0; k < 3; ++k) {
+            rs->kernel_ms[k] = kms[k];
+            rs->launches[k] = uint32_t(nchunks * max_depth);
+        }
+    }
+    return rc;
+}
 
 // Renders every job of the plan on `stream`. Returns PRT_OK or an error.
 int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::vector<WfJob>& jobs,
                   const std::vector<PlanJob>& pjobs, uint32_t max_depth, uint32_t capacity, uint32_t blocks_per_cu,
-                  bool count, RunStats* rs) {
+                  bool count, RunStats* rs, bool profile = false) {
     if (max_depth == 0 || max_depth > uint32_t(kMaxIter)) return fail(PRT_ERR_INVALID_ARG, "max_depth out of range");
     // Queue entries pack pixel | job << 24.
     if (jobs.size() > 256) return fail(PRT_ERR_LIMIT, "more than 256 jobs in one batch");
@@ -625,25 +700,17 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     const int gblocks = prop.multiProcessorCount * 8;
     const int sblocks = prop.multiProcessorCount * 2;  // 2 x 8 waves of shade per compute unit
 
+    if (profile) {
+        rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, rs);
+        if (rc == PRT_OK && rs) WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
+        return rc;
+    }
     // Capture one chunk's launch sequence.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WF_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(dev::wf_setup, dim3(1), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(dev::wf_generate, dim3(gblocks), dim3(256), 0, stream, a);
-    for (uint32_t it = 0; it < max_depth; ++it) {
-        if (count) {
-            hipLaunchKernelGGL(dev::wf_extend<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shade<true>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shadow<true>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-        } else {
-            hipLaunchKernelGGL(dev::wf_extend<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shade<false>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
-            hipLaunchKernelGGL(dev::wf_shadow<false>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
-        }
-    }
-    hipLaunchKernelGGL(dev::wf_resolve, dim3(gblocks), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(dev::wf_advance, dim3(1), dim3(64), 0, stream, a);
+    if (count) launch_chunk<true>(a, stream, max_depth, pblocks, gblocks, sblocks, nullptr);
+    else launch_chunk<false>(a, stream, max_depth, pblocks, gblocks, sblocks, nullptr);
     hipError_t e = hipStreamEndCapture(stream, &graph);
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -667,6 +734,10 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     if (e != hipSuccess) return hip_fail(e, "wavefront launch");
     if (rs) {
         rs->ms = ms;
+        for (int k = 0; k < 3; ++k) {
+            rs->kernel_ms[k] = 0.0;
+            rs->launches[k] = 0;
+        }
         WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
     }
     return PRT_OK;
@@ -694,7 +765,7 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
     RunStats rs;
     int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap, 0,
-                           r->cfg.count_rays != 0, &rs);
+                           r->cfg.count_rays != 0, &rs, r->cfg.profile_kernels != 0);
     if (rc != PRT_OK) return rc;
     // Fold the wavefront's counters into the renderer's.
     unsigned long long host[kCntNum];
@@ -702,6 +773,10 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     for (int k = 0; k < kCntNum; ++k) host[k] += rs.c[k];
     WF_TRY(hipMemcpy(r->counters, host, sizeof host, hipMemcpyHostToDevice));
     r->last_ms = rs.ms;
+    for (int k = 0; k < 3; ++k) {
+        r->kernel_ms[k] = rs.kernel_ms[k];
+        r->kernel_launches[k] = rs.launches[k];
+    }
     r->samples_done += n;
     r->samples_total += total;
     return PRT_OK;

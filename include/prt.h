@@ -119,6 +119,8 @@ typedef struct prt_render_config {
     uint32_t count_rays;    /* 1 = keep per-kind ray / node / triangle counters */
     uint32_t wavefront_chunk;   /* wavefront: samples per pixel resolved per chunk (0 = choose) */
     uint32_t wavefront_queue;   /* wavefront: path slots in flight (0 = choose) */
+    uint32_t profile_kernels;   /* wavefront: 1 = launch each chunk's kernels without a graph and
+                                   time every extend / shade / shadow launch with events */
 } prt_render_config;
 
 /* Counters accumulated since the renderer was created (or reset). */
@@ -131,6 +133,11 @@ typedef struct prt_stats {
     uint64_t hits;            /* closest hits shaded (64-byte shading records fetched) */
     uint64_t samples;         /* pixel-samples completed */
     double last_render_ms;    /* device time of the last prt_render_spp / batch call */
+    uint64_t anyhit_nodes;    /* wavefront: the part of nodes_visited fetched by shadow rays */
+    uint64_t anyhit_tris;     /* wavefront: the part of tris_tested fetched by shadow rays */
+    double kernel_ms[3];      /* profile_kernels: device time of the extend, shade and shadow
+                                 launches of the last wavefront render */
+    uint32_t kernel_launches[3];
 } prt_stats;
 
 /* BVH statistics of a built scene. */
