@@ -73,6 +73,8 @@ def parse():
     p.add_argument("--threads", type=int, default=0)
     p.add_argument("--frames-only", action="store_true",
                    help="only the timed frames of the headline integrator (for profilers)")
+    p.add_argument("--direct", action="store_true",
+                   help="with --frames-only: launch the wavefront chunks' kernels directly, not through graphs")
     p.add_argument("--copy-probe", action="store_true",
                    help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
     return p.parse_args()
@@ -199,7 +201,8 @@ def main():
     per = max(1, a.spp_per_launch)
     base = dict(camera=wl.camera, max_depth=a.depth, seed=W.SEED, spp_per_launch=per)
 
-    r = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ, **base)
+    r = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ,
+                            profile_kernels=int(a.frames_only and a.direct), **base)
     if a.frames_only:
         wall, dev_ms = timed_frames(r, a.spp, a.warmup, a.steps)
         print(json.dumps({"frames": a.steps, "ms_per_frame": round(dev_ms, 3)}))
