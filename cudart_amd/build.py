@@ -92,7 +92,7 @@ def build_cli() -> Path:
 
 
 def build_oracle() -> Path:
-    """The CPU oracle: liboracle.so and the `prt_cpu` CLI (test infrastructure)."""
+    """The CPU oracle: liboracle.so (test infrastructure)."""
     ORACLE_BUILD.mkdir(parents=True, exist_ok=True)
     lib = ORACLE_BUILD / "liboracle.so"
     tmp = ORACLE_BUILD / "liboracle.so.tmp"
@@ -101,16 +101,25 @@ def build_oracle() -> Path:
     return lib
 
 
+def build_oracle_cli() -> Path:
+    """`prt_cpu`: the product CLI's front end over the oracle's renderer (test infrastructure)."""
+    exe = ORACLE_BUILD / "prt_cpu"
+    _run([CXX, *CXX_FLAGS, str(CSRC / "host" / "cli.cpp"), str(ORACLE / "prt_cpu.cpp"), "-o", str(exe),
+          f"-L{LIB_DIR}", f"-L{ORACLE_BUILD}", "-lprt", "-loracle",
+          f"-Wl,-rpath,$ORIGIN/../../cudart_amd/lib", "-Wl,-rpath,$ORIGIN"])
+    return exe
+
+
 def build_all() -> None:
     with cf.ThreadPoolExecutor(max_workers=2) as ex:
         f_lib = ex.submit(build_library)
         f_orc = ex.submit(build_oracle)
         f_lib.result()
         f_orc.result()
-    if (CSRC / "host" / "cli.cpp").exists():
-        build_cli()
+    build_cli()
+    build_oracle_cli()
 
 
 if __name__ == "__main__":
     build_all()
-    print("built", LIB_DIR / "libprt.so", ORACLE_BUILD / "liboracle.so")
+    print("built", LIB_DIR / "libprt.so", BIN_DIR / "prt", ORACLE_BUILD / "liboracle.so", ORACLE_BUILD / "prt_cpu")

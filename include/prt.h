@@ -169,15 +169,25 @@ int prt_device_count(int *count); /* cudaGetDeviceCount, src/main.cpp:155 */
  * newmtl name (fixes the value-dedupe of scene.cpp:82-97,192-203). */
 int prt_mesh_load_obj(const char *obj_path, prt_mesh **out);
 
-/* Procedural scenes from a seed. kind: "cornell-box" (the obj fixture is loaded
- * separately), "spheres" (displaced icosphere instances, the 1M-triangle scene),
- * "cornell-spheres" (Cornell-like box with spheres), "terrain-spheres" (large
- * textured scene). params: key=value pairs separated by ';' (see DESIGN). */
+/* Procedural scenes from a seed (every draw from Philox keyed by seed and a stream id).
+ * kind:
+ *   "spheres"          displaced level-`level` icospheres (instances=200, radius=1,
+ *                      disp=0.05, spread=10, materials=32, ground=100, emission=15):
+ *                      the 1M-triangle scene;
+ *   "cornell-spheres"  Cornell-like box of displaced UV spheres (tris=100000,
+ *                      materials=16, emission=15);
+ *   "mixed"            the "spheres" scene sized to about `tris` triangles (tris=100000);
+ *   "terrain-spheres"  fBm height field (grid=1415, extent=200) plus icospheres
+ *                      (instances=2000) with RGBA8 textures (textures=16, texsize=1024,
+ *                      materials=64).
+ * width/height set the aspect of the stored camera. params: key=value pairs
+ * separated by ';' overriding the defaults above. */
 int prt_mesh_generate(const char *kind, const char *params, uint32_t seed, prt_mesh **out);
 
 /* Fills desc with pointers into the mesh (valid until prt_mesh_destroy). */
 int prt_mesh_get_desc(const prt_mesh *mesh, prt_scene_desc *desc);
-/* Replaces material i of the mesh (e.g. to force an emitter, BASE of config C1). */
+/* Replaces material i of the mesh (e.g. to force the ceiling quad of cornell.obj to
+ * be an emitter: its MTL has no `type`, so it loads as diffuse). */
 int prt_mesh_set_material(prt_mesh *mesh, uint32_t index, const prt_material *m);
 /* Copies the name of material i (the MTL newmtl name) into buf. */
 int prt_mesh_material_name(const prt_mesh *mesh, uint32_t index, char *buf, size_t cap);

@@ -54,3 +54,48 @@ def quad(a, b, c, d):
     pos = np.array([[a, b, c], [a, c, d]], np.float32)
     nrm = np.array([[n, n, n], [n, n, n]], np.float32)
     return pos, nrm
+
+
+def read_png_rgba(path) -> np.ndarray:
+    """Decodes an 8-bit RGBA, non-interlaced PNG (all five row filters) to (H, W, 4) uint8."""
+    import struct
+    import zlib
+
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n", "not a PNG"
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n, typ = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        assert zlib.crc32(typ + body) & 0xFFFFFFFF == crc, "bad CRC"
+        if typ == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, depth, ctype, _, _, interlace = hdr
+    assert depth == 8 and ctype == 6 and interlace == 0, hdr
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    out = np.zeros((h, 4 * w), np.int32)
+    for y in range(h):
+        f, row = raw[y, 0], raw[y, 1:].astype(np.int32)
+        prev = out[y - 1] if y else np.zeros(4 * w, np.int32)
+        cur = out[y]
+        for x in range(4 * w):
+            a = cur[x - 4] if x >= 4 else 0
+            b = prev[x]
+            c = prev[x - 4] if x >= 4 else 0
+            if f == 0:
+                p = 0
+            elif f == 1:
+                p = a
+            elif f == 2:
+                p = b
+            elif f == 3:
+                p = (a + b) // 2
+            else:
+                pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            cur[x] = (row[x] + p) & 0xFF
+    return out.astype(np.uint8).reshape(h, w, 4)
