@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
                                   C.POINTER(Cfg), DP, U64P, C.c_int],
             "oracle_render_pixels_f32": [P, C.POINTER(Camera), C.c_uint32, C.c_uint32, UP, C.c_uint32, C.c_uint32,
                                          C.c_uint32, C.POINTER(Cfg), FP, U64P, C.c_int],
+            "oracle_render_pixels_f64": [P, C.POINTER(Camera), C.c_uint32, C.c_uint32, UP, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.POINTER(Cfg), DP, U64P, C.c_int],
             "oracle_first_hit": [P, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Cfg), IP, IP,
                                  FP, C.c_int, C.c_int],
             "oracle_trace": [P, C.c_uint32, FP, FP, IP, FP, FP, FP, C.c_int, C.c_int],
@@ -121,12 +123,19 @@ class OracleScene:
         _chk(rc, "oracle_render")
         return accum, cnt
 
-    def render_pixels(self, cam: Camera, w: int, h: int, pixels, s0: int, n: int, c: Cfg, threads: int = 0):
+    def render_pixels(self, cam: Camera, w: int, h: int, pixels, s0: int, n: int, c: Cfg, threads: int = 0,
+                      dtype=np.float32):
+        """Sum of samples s0..s0+n-1 of each listed pixel (index y*w+x), in fp32 or fp64."""
         px = np.ascontiguousarray(pixels, np.uint32)
-        out = np.zeros((len(px), 3), np.float32)
+        out = np.zeros((len(px), 3), dtype)
         cnt = np.zeros(3, np.uint64)
-        _chk(lib().oracle_render_pixels_f32(self._h, C.byref(cam), w, h, px.ctypes.data_as(UP), len(px), s0, n,
-                                            C.byref(c), _f(out), cnt.ctypes.data_as(U64P), threads), "render_pixels")
+        if out.dtype == np.float32:
+            rc = lib().oracle_render_pixels_f32(self._h, C.byref(cam), w, h, px.ctypes.data_as(UP), len(px), s0, n,
+                                                C.byref(c), _f(out), cnt.ctypes.data_as(U64P), threads)
+        else:
+            rc = lib().oracle_render_pixels_f64(self._h, C.byref(cam), w, h, px.ctypes.data_as(UP), len(px), s0, n,
+                                                C.byref(c), out.ctypes.data_as(DP), cnt.ctypes.data_as(U64P), threads)
+        _chk(rc, "render_pixels")
         return out, cnt
 
     def first_hit(self, cam: Camera, w: int, h: int, sample: int, c: Cfg, brute: bool = False, threads: int = 0):

@@ -859,6 +859,38 @@ void adjointPixels(const oracle_scene* s, const prt_camera* cam, uint32_t w, uin
 // ===========================================================================
 // C entry points
 // ===========================================================================
+template <class T>
+int renderPixelsImpl(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h, const uint32_t* pixels,
+                     uint32_t count, uint32_t s0, uint32_t n, const oracle_cfg* cfg, T* rgb, uint64_t counters[3],
+                     int threads) {
+    if (!s || !cam || !cfg || !pixels || !rgb) return PRT_ERR_INVALID_ARG;
+    if (s->lights.empty()) return PRT_ERR_NO_EMITTERS;
+    if (int64_t(w) != cam->xpixels || int64_t(h) != cam->ypixels) return PRT_ERR_INVALID_ARG;
+    for (uint32_t i = 0; i < count; ++i)
+        if (pixels[i] >= w * h) return PRT_ERR_INVALID_ARG;
+    uint64_t cp = 0, ce = 0, cs = 0;
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : cp, ce, cs)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+        Counters c;
+        int x = int(pixels[i] % w), y = int(pixels[i] / w);
+        V3<T> acc{0, 0, 0};
+        for (uint32_t k = 0; k < n; ++k) acc = acc + samplePixel<T, false>(*s, *cam, *cfg, x, y, s0 + k, c, nullptr);
+        rgb[i * 3] = acc.x;
+        rgb[i * 3 + 1] = acc.y;
+        rgb[i * 3 + 2] = acc.z;
+        cp += c.primary;
+        ce += c.extension;
+        cs += c.shadow;
+    }
+    if (counters) {
+        counters[0] += cp;
+        counters[1] += ce;
+        counters[2] += cs;
+    }
+    return PRT_OK;
+}
+
 extern "C" {
 
 int oracle_scene_create(const prt_scene_desc* d, oracle_scene** out) {
@@ -1018,33 +1050,15 @@ int oracle_render_f64(const oracle_scene* s, const prt_camera* cam, uint32_t w, 
 int oracle_render_pixels_f32(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h,
                              const uint32_t* pixels, uint32_t count, uint32_t s0, uint32_t n,
                              const oracle_cfg* cfg, float* rgb, uint64_t counters[3], int threads) {
-    if (!s || !cam || !cfg || !pixels || !rgb) return PRT_ERR_INVALID_ARG;
-    if (s->lights.empty()) return PRT_ERR_NO_EMITTERS;
-    if (int64_t(w) != cam->xpixels || int64_t(h) != cam->ypixels) return PRT_ERR_INVALID_ARG;
-    for (uint32_t i = 0; i < count; ++i)
-        if (pixels[i] >= w * h) return PRT_ERR_INVALID_ARG;
-    uint64_t cp = 0, ce = 0, cs = 0;
-    int nt = threads > 0 ? threads : omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : cp, ce, cs)
-    for (int64_t i = 0; i < int64_t(count); ++i) {
-        Counters c;
-        int x = int(pixels[i] % w), y = int(pixels[i] / w);
-        V3<float> acc{0, 0, 0};
-        for (uint32_t k = 0; k < n; ++k) acc = acc + samplePixel<float, false>(*s, *cam, *cfg, x, y, s0 + k, c, nullptr);
-        rgb[i * 3] = acc.x;
-        rgb[i * 3 + 1] = acc.y;
-        rgb[i * 3 + 2] = acc.z;
-        cp += c.primary;
-        ce += c.extension;
-        cs += c.shadow;
-    }
-    if (counters) {
-        counters[0] += cp;
-        counters[1] += ce;
-        counters[2] += cs;
-    }
-    return PRT_OK;
+    return renderPixelsImpl<float>(s, cam, w, h, pixels, count, s0, n, cfg, rgb, counters, threads);
 }
+
+int oracle_render_pixels_f64(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h,
+                             const uint32_t* pixels, uint32_t count, uint32_t s0, uint32_t n,
+                             const oracle_cfg* cfg, double* rgb, uint64_t counters[3], int threads) {
+    return renderPixelsImpl<double>(s, cam, w, h, pixels, count, s0, n, cfg, rgb, counters, threads);
+}
+
 
 int oracle_first_hit(const oracle_scene* s, const prt_camera* cam, uint32_t w, uint32_t h, uint32_t sample,
                      const oracle_cfg* cfg, int32_t* tri, int32_t* mat, float* t, int brute_force, int threads) {

@@ -51,6 +51,10 @@ int oracle_render_pixels_f32(const oracle_scene *s, const prt_camera *cam, uint3
                              const uint32_t *pixels, uint32_t count, uint32_t s0, uint32_t n,
                              const oracle_cfg *cfg, float *rgb, uint64_t counters[3], int threads);
 
+int oracle_render_pixels_f64(const oracle_scene *s, const prt_camera *cam, uint32_t w, uint32_t h,
+                             const uint32_t *pixels, uint32_t count, uint32_t s0, uint32_t n,
+                             const oracle_cfg *cfg, double *rgb, uint64_t counters[3], int threads);
+
 /* Primary-ray closest hit per pixel for one sample: original triangle id, material id, t. */
 int oracle_first_hit(const oracle_scene *s, const prt_camera *cam, uint32_t w, uint32_t h,
                      uint32_t sample, const oracle_cfg *cfg, int32_t *tri, int32_t *mat, float *t,

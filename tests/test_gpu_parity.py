@@ -112,3 +112,21 @@ def test_image_bytes_match_oracle_tonemap(gpu):
     rgba = r.copyImageBytes()
     ref = O.tonemap(r.radiance()).reshape(48, 64, 4)
     assert np.array_equal(rgba, ref)
+
+
+def test_radiance_textured_terrain_matches_oracle(gpu):
+    # Small version of the large scene: fBm terrain + icospheres with UV-mapped RGBA8
+    # textures (software bilinear with wrap on both sides).
+    wl = W.large(160, 90, 4, params="grid=120;instances=40;textures=4;texsize=64;materials=12")
+    assert any(m.tex_kd >= 0 for m in wl.mesh.materials)
+    _, r, orc, c = _pair(wl, count_rays=1)
+    r.render(wl.spp)
+    img = r.radiance()
+    acc, cnt = orc.render(wl.camera, wl.width, wl.height, 0, wl.spp, c)
+    oimg = acc / np.float32(wl.spp)
+    np.testing.assert_allclose(img, oimg, rtol=RTOL, atol=ATOL)
+    st = r.stats()
+    assert (st.primary_rays, st.extension_rays, st.shadow_rays) == tuple(int(x) for x in cnt)
+    tri, mat, _ = r.first_hit(0)
+    textured = [i for i, m in enumerate(wl.mesh.materials) if m.tex_kd >= 0]
+    assert np.isin(mat, textured).mean() > 0.05
