@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
                                   C.c_float, FP, FP],
             "oracle_tonemap": [C.c_uint32, FP, BP],
             "oracle_texture_table": [FP],
+            "oracle_texture_sample": [C.c_void_p, C.c_uint32, DP, DP, C.c_int],
             "oracle_adjoint": [P, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Cfg), FP, FP, DP,
                                DP, C.c_double, C.c_int, C.c_int, C.c_int],
             "oracle_scene_set_material_params": [P, C.c_uint32, FP, C.c_float],
@@ -250,3 +251,18 @@ def texture_table():
     t = np.zeros(256, np.float32)
     _chk(lib().oracle_texture_table(_f(t)), "texture_table")
     return t
+
+
+def texture_sample(rgba, uv, fp64: bool = True):
+    """Bilinear, wrapped lookup of an (H, W, 4) uint8 texture at (u, v) pairs -> (n, 3)."""
+    from cudart_amd._capi import Texture
+
+    img = np.ascontiguousarray(rgba, np.uint8)
+    t = Texture()
+    t.width, t.height = img.shape[1], img.shape[0]
+    t.rgba = img.ctypes.data_as(BP)
+    q = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    out = np.zeros((len(q), 3), np.float64)
+    _chk(lib().oracle_texture_sample(C.byref(t), len(q), q.ctypes.data_as(DP), out.ctypes.data_as(DP), int(fp64)),
+         "texture_sample")
+    return out

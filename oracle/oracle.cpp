@@ -1189,6 +1189,26 @@ int oracle_tonemap(uint32_t n, const float* rgb, uint8_t* rgba) {
     return PRT_OK;
 }
 
+int oracle_texture_sample(const prt_texture* tex, uint32_t n, const double* uv, double* rgb, int fp64) {
+    if (!tex || !tex->rgba || tex->width == 0 || tex->height == 0 || !uv || !rgb) return PRT_ERR_INVALID_ARG;
+    Tex tx;
+    tx.w = tex->width;
+    tx.h = tex->height;
+    tx.lin.resize(size_t(tx.w) * tx.h * 3);
+    for (size_t p = 0; p < size_t(tx.w) * tx.h; ++p)
+        for (int k = 0; k < 3; ++k) tx.lin[p * 3 + k] = std::pow(float(tex->rgba[p * 4 + k]) / 255.0f, 2.2f) * 1.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (fp64) {
+            V3<double> c = texSample<double>(tx, uv[2 * i], uv[2 * i + 1]);
+            rgb[3 * i] = c.x, rgb[3 * i + 1] = c.y, rgb[3 * i + 2] = c.z;
+        } else {
+            V3<float> c = texSample<float>(tx, float(uv[2 * i]), float(uv[2 * i + 1]));
+            rgb[3 * i] = c.x, rgb[3 * i + 1] = c.y, rgb[3 * i + 2] = c.z;
+        }
+    }
+    return PRT_OK;
+}
+
 int oracle_texture_table(float out[256]) {
     for (int i = 0; i < 256; ++i) out[i] = std::pow(float(i) / 255.0f, 2.2f) * 1.0f;
     return PRT_OK;

@@ -75,6 +75,8 @@ int oracle_camera_ray(const prt_camera *cam, int32_t x, int32_t y, float u0, flo
                       float o[3], float d[3]);
 int oracle_tonemap(uint32_t n, const float *rgb, uint8_t *rgba);
 int oracle_texture_table(float out[256]);
+/* Bilinear texture lookup with wrap (the kernels' tex_sample) of n (u, v) pairs. */
+int oracle_texture_sample(const prt_texture *tex, uint32_t n, const double *uv, double *rgb, int fp64);
 
 /* Path-replay adjoint: gradients of sum (I - T)^2 / (W*H*3) with respect to every
  * material's kd (RGB) and emission scale, accumulated in fp64. The replayed paths use

@@ -237,3 +237,33 @@ def test_f32_and_f64_oracles_agree():
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
     assert np.median(rel) < 1e-5
     assert np.mean(rel < 1e-3) > 0.97
+
+
+def test_texture_sampling_identities():
+    # Bilinear with wrap on both axes, texel (i, j) centred at ((i + .5) / W, (j + .5) / H),
+    # rows indexed by v. Checked by identities a wrong axis, offset or wrap would break.
+    rng = np.random.default_rng(4)
+    H, W = 5, 7
+    img = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    lin = O.texture_table()[img[..., :3]].astype(np.float64)  # the gamma-2.2 table, pinned above
+    # texel centres return the texel
+    ij = [(i, j) for j in range(H) for i in range(W)]
+    uv = np.array([((i + 0.5) / W, (j + 0.5) / H) for i, j in ij])
+    got = O.texture_sample(img, uv)
+    np.testing.assert_allclose(got, np.array([lin[j, i] for i, j in ij]), rtol=1e-12, atol=1e-12)
+    # half-way between horizontal neighbours: their mean, including the wrap from W-1 to 0
+    uv = np.array([((i + 1.0) / W, (j + 0.5) / H) for i, j in ij])
+    want = np.array([(lin[j, i] + lin[j, (i + 1) % W]) / 2 for i, j in ij])
+    np.testing.assert_allclose(O.texture_sample(img, uv), want, rtol=1e-12, atol=1e-12)
+    # and vertical neighbours
+    uv = np.array([((i + 0.5) / W, (j + 1.0) / H) for i, j in ij])
+    want = np.array([(lin[j, i] + lin[(j + 1) % H, i]) / 2 for i, j in ij])
+    np.testing.assert_allclose(O.texture_sample(img, uv), want, rtol=1e-12, atol=1e-12)
+    # periodic in u and v; a constant texture is reproduced; fp32 within rounding of fp64
+    q = rng.uniform(-3, 3, (200, 2))
+    base = O.texture_sample(img, q)
+    np.testing.assert_allclose(O.texture_sample(img, q + [1.0, -2.0]), base, rtol=1e-9, atol=1e-12)
+    const = np.full((H, W, 4), 200, np.uint8)
+    c = float(O.texture_table()[200])
+    np.testing.assert_allclose(O.texture_sample(const, q), np.full((200, 3), c), rtol=1e-12)
+    np.testing.assert_allclose(O.texture_sample(img, q, fp64=False), base, rtol=1e-5, atol=1e-6)
