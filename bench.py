@@ -17,7 +17,7 @@ is the same), and the ratio of the two rates is reported.
 
 Roofline of the dominant kernel (wf_extend, closest-hit traversal of every primary
 and extension ray): algorithmic bytes per launch
-    64 B x child-pair nodes + 48 B x triangle records + 64 B x shading records
+    112 B x four-wide nodes + 48 B x triangle records + 64 B x shading records
 of the closest-hit rays (device counters of the instrumented frame, shadow-ray
 share removed), divided by the average extend launch time measured with HIP events
 around every extend launch on the renderer's stream (a frame rendered with the
@@ -196,6 +196,7 @@ def main():
     scene = ca.Scene(wl.mesh)
     build_s = time.perf_counter() - t0
     info = scene.bvh_info()
+    nb = info.node_read_bytes  # bytes one node visit reads
     wave = a.integrator == "wavefront"
     integ = ca.INTEGRATOR_WAVEFRONT if wave else ca.INTEGRATOR_MEGAKERNEL
     per = max(1, a.spp_per_launch)
@@ -229,7 +230,8 @@ def main():
         rp.close()
         launches = sp.kernel_launches[0]
         launch_ms = sp.kernel_ms[0] / launches
-        ext_bytes = 64 * (st.nodes_visited - st.anyhit_nodes) + 48 * (st.tris_tested - st.anyhit_tris) + 64 * st.hits
+        ext_bytes = (nb * (st.nodes_visited - st.anyhit_nodes) + 48 * (st.tris_tested - st.anyhit_tris)
+                     + 64 * st.hits)
         bytes_per_launch = ext_bytes / launches
         kernel = "wf_extend"
         kernel_times = {"extend": round(sp.kernel_ms[0], 2), "shade": round(sp.kernel_ms[1], 2),
@@ -240,7 +242,7 @@ def main():
     else:
         launches = (a.spp + per - 1) // per
         launch_ms = dev_ms / launches
-        bytes_per_launch = (64 * st.nodes_visited + 48 * st.tris_tested + 64 * st.hits) / launches
+        bytes_per_launch = (nb * st.nodes_visited + 48 * st.tris_tested + 64 * st.hits) / launches
         kernel = "megakernel"
         kernel_times = None
         per_ray = {"nodes": round(st.nodes_visited / rays, 2), "tris": round(st.tris_tested / rays, 2)}
@@ -271,8 +273,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
                      "launch_ms": round(launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
                      "per_ray": per_ray, "kernel_ms_per_frame": kernel_times},
-        "bvh": {"nodes": info.num_nodes, "max_depth": info.max_depth, "sah_cost": round(info.sah_cost, 2),
-                "build_s": round(build_s, 2)},
+        "bvh": {"nodes": info.num_nodes, "node_read_bytes": nb, "max_depth": info.max_depth,
+                "sah_cost": round(info.sah_cost, 2), "build_s": round(build_s, 2)},
     }
     if wave and not a.no_compare:
         rm = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=ca.INTEGRATOR_MEGAKERNEL, **base)

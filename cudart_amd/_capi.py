@@ -61,7 +61,7 @@ class Stats(C.Structure):
 class BvhInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("num_leaves", C.c_uint32), ("max_depth", C.c_uint32),
                 ("max_leaf_size", C.c_uint32), ("node_bytes", C.c_uint64), ("triangle_bytes", C.c_uint64),
-                ("shading_bytes", C.c_uint64), ("sah_cost", C.c_float)]
+                ("shading_bytes", C.c_uint64), ("sah_cost", C.c_float), ("node_read_bytes", C.c_uint32)]
 
 
 class Job(C.Structure):

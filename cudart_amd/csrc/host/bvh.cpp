@@ -167,6 +167,7 @@ void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth
     }
 }
 
+// Writes the four-wide layout depth first from the binary build tree.
 struct Emitter {
     const BNode* bn;
     const float (*pbox)[6];
@@ -176,41 +177,54 @@ struct Emitter {
     double sah = 0.0;
     double root_area = 1.0;
 
+    bool inner(int32_t b) const { return bn[b].child[0] >= 0; }
+    double area(int32_t b) const { return double(box_area(pbox[b], pbox[b] + 3)); }
+
+    // Up to four build nodes that replace inner build node b: its children, with
+    // the largest inner one opened until there are four.
+    int gather(int32_t b, int32_t slot[4]) const {
+        int n = 2;
+        slot[0] = bn[b].child[0];
+        slot[1] = bn[b].child[1];
+        while (n < 4) {
+            int best = -1;
+            for (int k = 0; k < n; ++k)
+                if (inner(slot[k]) && (best < 0 || area(slot[k]) > area(slot[best]))) best = k;
+            if (best < 0) break;
+            const int32_t open = slot[best];
+            slot[best] = bn[open].child[0];
+            slot[n++] = bn[open].child[1];
+        }
+        return n;
+    }
     int32_t child_word(int32_t b, uint32_t depth) {
         const BNode& c = bn[b];
         if (c.child[0] < 0) {
             leaves++;
             max_leaf = std::max(max_leaf, c.count);
-            sah += double(box_area(pbox[b], pbox[b] + 3)) / root_area * double(c.count);
+            sah += area(b) / root_area * double(c.count);
             return leaf_word(c.start, c.count);
         }
         uint32_t k = next++;
         emit(b, k, depth + 1);
         return int32_t(k);
     }
-    void write_box(uint32_t k, int slot, const float* bx) {
-        float* nd = &out[size_t(k) * 16];
-        // bx = lo[3], hi[3]
-        nd[slot * 4 + 0] = bx[0];
-        nd[slot * 4 + 1] = bx[3];
-        nd[slot * 4 + 2] = bx[1];
-        nd[slot * 4 + 3] = bx[4];
-        nd[8 + slot * 2 + 0] = bx[2];
-        nd[8 + slot * 2 + 1] = bx[5];
-    }
     void emit(int32_t b, uint32_t k, uint32_t depth) {
         max_depth = std::max(max_depth, depth);
-        sah += double(box_area(pbox[b], pbox[b] + 3)) / root_area;
-        const BNode& nd = bn[b];
-        int32_t w0 = child_word(nd.child[0], depth);
-        int32_t w1 = child_word(nd.child[1], depth);
-        write_box(k, 0, pbox[nd.child[0]]);
-        write_box(k, 1, pbox[nd.child[1]]);
-        float* o = &out[size_t(k) * 16];
-        std::memcpy(&o[12], &w0, 4);
-        std::memcpy(&o[13], &w1, 4);
-        o[14] = 0.f;
-        o[15] = 0.f;
+        sah += area(b) / root_area;
+        int32_t slot[4];
+        const int n = gather(b, slot);
+        int32_t words[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+        for (int c = 0; c < n; ++c) words[c] = child_word(slot[c], depth);
+        float* o = &out[size_t(k) * kNodeFloats];
+        for (int c = 0; c < 4; ++c) {
+            const float* bx = c < n ? pbox[slot[c]] : nullptr;  // lo[3], hi[3]
+            for (int axis = 0; axis < 3; ++axis) {
+                o[axis * 8 + c] = bx ? bx[axis] : 0.f;
+                o[axis * 8 + 4 + c] = bx ? bx[3 + axis] : 0.f;
+            }
+            std::memcpy(&o[24 + c], &words[c], 4);
+        }
     }
 };
 
@@ -279,33 +293,26 @@ void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout&
     for (uint32_t i = 0; i < total; ++i) inner += bn[i].child[0] >= 0 ? 1u : 0u;
     out = BvhLayout();
     if (inner == 0) {
-        // Whole scene in one leaf: root pairs it with an empty child whose box is inverted.
-        out.nodes.assign(16, 0.f);
+        // Whole scene in one leaf: the root holds it in slot 0, the other slots empty.
+        out.nodes.assign(kNodeFloats, 0.f);
         float* o = out.nodes.data();
-        const float* b = &pb[0];
-        float empty[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-        auto put = [&](int slot, const float* bx) {
-            o[slot * 4 + 0] = bx[0];
-            o[slot * 4 + 1] = bx[3];
-            o[slot * 4 + 2] = bx[1];
-            o[slot * 4 + 3] = bx[4];
-            o[8 + slot * 2 + 0] = bx[2];
-            o[8 + slot * 2 + 1] = bx[5];
-        };
-        put(0, b);
-        put(1, empty);
-        int32_t w0 = leaf_word(0, n), w1 = leaf_word(0, 1);
-        std::memcpy(&o[12], &w0, 4);
-        std::memcpy(&o[13], &w1, 4);
+        for (int axis = 0; axis < 3; ++axis) {
+            o[axis * 8] = pb[axis];
+            o[axis * 8 + 4] = pb[3 + axis];
+        }
+        const int32_t words[4] = {leaf_word(0, n), kEmptyChild, kEmptyChild, kEmptyChild};
+        std::memcpy(&o[24], words, sizeof words);
         out.num_nodes = 1;
         out.num_leaves = 1;
         out.max_leaf = n;
         out.sah_cost = float(n);
     } else {
-        out.nodes.assign(size_t(inner) * 16, 0.f);
+        // At most `inner` four-wide nodes (each consumes at least one inner build node).
+        out.nodes.assign(size_t(inner) * kNodeFloats, 0.f);
         Emitter em{bn.data(), reinterpret_cast<const float(*)[6]>(pb.data()), out.nodes};
         em.root_area = std::max(1e-30, double(box_area(&pb[0], &pb[3])));
         em.emit(0, 0, 0);
+        out.nodes.resize(size_t(em.next) * kNodeFloats);
         out.num_nodes = em.next;
         out.num_leaves = em.leaves;
         out.max_depth = em.max_depth;

@@ -9,13 +9,20 @@
 
 namespace prt {
 
-// Child-pair node, 64 bytes = 16 floats:
-//   [0..3]   c0.lo.x c0.hi.x c0.lo.y c0.hi.y
-//   [4..7]   c1.lo.x c1.hi.x c1.lo.y c1.hi.y
-//   [8..11]  c0.lo.z c0.hi.z c1.lo.z c1.hi.z
-//   [12..15] child0 child1 (int32 bit patterns), 0, 0
-// A child word >= 0 is an inner node index. A negative word is a leaf:
+// Four-wide node, 128 bytes = 32 floats (one 128-byte line; a visit reads the
+// first 112 bytes), boxes of the four children as structure of arrays:
+//   [0..3]   lo.x of children 0..3     [4..7]   hi.x
+//   [8..11]  lo.y                      [12..15] hi.y
+//   [16..19] lo.z                      [20..23] hi.z
+//   [24..27] child words (int32 bit patterns)      [28..31] 0
+// A child word >= 0 is an inner node index, kEmptyChild an unused slot (its box
+// is zero). A negative word is a leaf:
 //   bit 31 set, bits 28..30 = count-1 (1..8 triangles), bits 0..27 = first record.
+// Collapsed from a binary SAH tree: a node takes its two children and keeps
+// opening the inner child with the largest box area until it has four.
+constexpr int kNodeFloats = 32;
+constexpr uint32_t kNodeReadBytes = 112;
+constexpr int32_t kEmptyChild = 0x7FFFFFFF;
 constexpr int kMaxLeaf = 8;
 constexpr uint32_t kMaxRecords = 1u << 28;
 constexpr int kLeafCountShift = 28;
@@ -24,11 +31,11 @@ inline int32_t leaf_word(uint32_t first, uint32_t count) {
 }
 
 struct BvhLayout {
-    std::vector<float> nodes;     // 16 floats per node
+    std::vector<float> nodes;     // kNodeFloats floats per node
     std::vector<uint32_t> order;  // record index -> original triangle id
     uint32_t num_nodes = 0;
     uint32_t num_leaves = 0;
-    uint32_t max_depth = 0;       // deepest child-pair node (root = 0)
+    uint32_t max_depth = 0;       // deepest four-wide node (root = 0)
     uint32_t max_leaf = 0;
     float sah_cost = 0.f;
 };

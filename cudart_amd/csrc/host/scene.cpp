@@ -80,7 +80,8 @@ extern "C" int prt_scene_create(const prt_scene_desc* d, prt_scene** out) {
 
     BvhParams bp;
     build_bvh(d->positions, n, bp, s->bvh);
-    if (s->bvh.max_depth + 2 > 128) return fail(PRT_ERR_LIMIT, "BVH deeper than the traversal stack");
+    // A visit pushes at most three siblings; the traversal stack holds 128 entries.
+    if (3 * (s->bvh.max_depth + 1) > 128) return fail(PRT_ERR_LIMIT, "BVH deeper than the traversal stack");
     const std::vector<uint32_t>& order = s->bvh.order;
     std::vector<uint32_t> where(n);
     for (uint32_t r = 0; r < n; ++r) where[order[r]] = r;
@@ -135,7 +136,8 @@ extern "C" int prt_scene_bvh_info(const prt_scene* s, prt_bvh_info* info) {
     info->num_leaves = s->bvh.num_leaves;
     info->max_depth = s->bvh.max_depth;
     info->max_leaf_size = s->bvh.max_leaf;
-    info->node_bytes = uint64_t(s->bvh.num_nodes) * 64;
+    info->node_bytes = uint64_t(s->bvh.num_nodes) * kNodeFloats * sizeof(float);
+    info->node_read_bytes = kNodeReadBytes;
     info->triangle_bytes = uint64_t(s->num_triangles) * 48;
     info->shading_bytes = uint64_t(s->num_triangles) * 64;
     info->sah_cost = s->bvh.sah_cost;

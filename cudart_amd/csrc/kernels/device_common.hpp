@@ -129,13 +129,36 @@ struct Counts {
 // ---- traversal -----------------------------------------------------------
 // Short stack in LDS (kStackLds entries per lane, lane-interleaved so any mix of
 // depths is bank-conflict free), continued in a global spill area.
+// Explicit address spaces: scene arrays reached through a pointer loaded from
+// memory are otherwise generic (flat) accesses, which take 64-bit VGPR addresses
+// and wait on the LDS counter as well.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+// p[i] as a global-memory load (the host compilation pass sees a plain load).
+template <class T>
+__device__ __forceinline__ T gload(const T* p, size_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((gptr<T>)p)[i];
+#else
+    return p[i];
+#endif
+}
+using lds_int = __attribute__((address_space(3))) int;
+using glob_int = __attribute__((address_space(1))) int;
+
 template <int LDS>
 struct StackT {
     static constexpr int kLds = LDS;
-    int* lds;              // this lane's column: lds[k * kBlock]
-    int* spill;            // this thread's column in the spill area
+    lds_int* lds;          // this lane's column: lds[k * kBlock]
+    glob_int* spill;       // this thread's column in the spill area
     uint32_t spill_stride; // threads sharing the spill area
     int sp;
+    __device__ __forceinline__ void bind(int* lds_col, int* spill_col, uint32_t stride) {
+        lds = (lds_int*)lds_col;
+        spill = (glob_int*)spill_col;
+        spill_stride = stride;
+        sp = 0;
+    }
     __device__ __forceinline__ void push(int v) {
         if (sp < LDS) lds[sp * kBlock] = v;
         else spill[size_t(sp - LDS) * spill_stride] = v;
@@ -143,7 +166,10 @@ struct StackT {
     }
     __device__ __forceinline__ int pop() {
         --sp;
-        return sp < LDS ? lds[sp * kBlock] : spill[size_t(sp - LDS) * spill_stride];
+        int v;
+        if (sp < LDS) v = lds[sp * kBlock];
+        else v = spill[size_t(sp - LDS) * spill_stride];
+        return v;
     }
 };
 using Stack = StackT<kStackLds>;
@@ -172,6 +198,53 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
     return t > kEps;
 }
 
+// One four-wide node: slab tests of the four child boxes against [tmin, tfar],
+// then the children hit, nearest first. Pushes the farther ones (farthest first,
+// so the nearest of them is popped next) and returns the nearest, or kNoChild.
+constexpr int kNoChild = 0x7FFFFFFF;  // also the word of an empty slot
+__device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    kb = sw ? ka : kb;
+    ka = k;
+    const int w = sw ? wb : wa;
+    wb = sw ? wa : wb;
+    wa = w;
+}
+template <class StackType>
+__device__ __forceinline__ int visit_node(const float4* nodes, int node, float ix, float iy, float iz, float ox,
+                                          float oy, float oz, float tmin, float tfar, StackType& st) {
+    const size_t nb = size_t(node) * 8;
+    const float4 lx = gload(nodes, nb), hx = gload(nodes, nb + 1), ly = gload(nodes, nb + 2),
+                 hy = gload(nodes, nb + 3), lz = gload(nodes, nb + 4), hz = gload(nodes, nb + 5);
+    const int4 w = gload(reinterpret_cast<const int4*>(nodes), nb + 6);
+    float key[4];
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    int word[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float a0 = __builtin_fmaf(lxa[c], ix, -ox), b0 = __builtin_fmaf(hxa[c], ix, -ox);
+        const float a1 = __builtin_fmaf(lya[c], iy, -oy), b1 = __builtin_fmaf(hya[c], iy, -oy);
+        const float a2 = __builtin_fmaf(lza[c], iz, -oz), b2 = __builtin_fmaf(hza[c], iz, -oz);
+        const float tn = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
+        const float tf = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tfar));
+        const bool hit = tn <= tf && word[c] != kNoChild;
+        key[c] = hit ? tn : __builtin_inff();
+        word[c] = hit ? word[c] : kNoChild;
+    }
+    cswap(key[0], word[0], key[1], word[1]);
+    cswap(key[2], word[2], key[3], word[3]);
+    cswap(key[0], word[0], key[2], word[2]);
+    cswap(key[1], word[1], key[3], word[3]);
+    cswap(key[1], word[1], key[2], word[2]);
+    if (word[3] != kNoChild) st.push(word[3]);
+    if (word[2] != kNoChild) st.push(word[2]);
+    if (word[1] != kNoChild) st.push(word[1]);
+    return word[0];
+}
+
 // Closest hit (ANY = false) or occlusion test against a known best (ANY = true).
 // Accepts t in (tmin, best.t) or t == best.t with a lower original id: the
 // closest hit is the lexicographic minimum of (t, original id), independent of
@@ -188,38 +261,16 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
     int node = 0;
     while (true) {
         if (node >= 0) {
-            const float4* np = sc.nodes + size_t(node) * 4;
-            float4 n0 = np[0], n1 = np[1], n2 = np[2];
-            int4 n3 = reinterpret_cast<const int4*>(np)[3];
             if (COUNT) cnt.nodes++;
-            float a0 = __builtin_fmaf(n0.x, ix, -ox), b0 = __builtin_fmaf(n0.y, ix, -ox);
-            float a1 = __builtin_fmaf(n0.z, iy, -oy), b1 = __builtin_fmaf(n0.w, iy, -oy);
-            float a2 = __builtin_fmaf(n2.x, iz, -oz), b2 = __builtin_fmaf(n2.y, iz, -oz);
-            float tn0 = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
-            float tf0 = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), best.t));
-            float c0 = __builtin_fmaf(n1.x, ix, -ox), d0 = __builtin_fmaf(n1.y, ix, -ox);
-            float c1 = __builtin_fmaf(n1.z, iy, -oy), d1 = __builtin_fmaf(n1.w, iy, -oy);
-            float c2 = __builtin_fmaf(n2.z, iz, -oz), d2 = __builtin_fmaf(n2.w, iz, -oz);
-            float tn1 = fmaxf(fmaxf(fminf(c0, d0), fminf(c1, d1)), fmaxf(fminf(c2, d2), tmin));
-            float tf1 = fminf(fminf(fmaxf(c0, d0), fmaxf(c1, d1)), fminf(fmaxf(c2, d2), best.t));
-            bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
-            if (h0 && h1) {
-                bool first0 = tn0 <= tn1;
-                st.push(first0 ? n3.y : n3.x);
-                node = first0 ? n3.x : n3.y;
-                continue;
-            }
-            if (h0 || h1) {
-                node = h0 ? n3.x : n3.y;
-                continue;
-            }
+            node = visit_node(sc.nodes, node, ix, iy, iz, ox, oy, oz, tmin, best.t, st);
+            if (node != kNoChild) continue;
         } else {
             const uint32_t w = uint32_t(node);
             const uint32_t first = w & 0x0FFFFFFFu;
             const uint32_t count = ((w >> 28) & 7u) + 1u;
             for (uint32_t k = 0; k < count; ++k) {
-                const float4* tp = sc.tris + size_t(first + k) * 3;
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                const size_t tb = size_t(first + k) * 3;
+                float4 a = gload(sc.tris, tb), b = gload(sc.tris, tb + 1), c = gload(sc.tris, tb + 2);
                 if (COUNT) cnt.tris++;
                 float t, u, v;
                 if (!mt_intersect(a, b, c, o, d, t, u, v)) continue;
@@ -244,6 +295,71 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
         node = st.pop();
     }
     return found;
+}
+
+// Resumable form of traverse() for kernels that refill finished lanes with new
+// rays: the same visiting order, culling and (t, id) ordering, one node or leaf
+// per call. trav_begin() sets up a ray; trav_step() advances it and returns
+// kTravRunning, kTravDone (closest hit: best holds the result; any-hit: no
+// occluder) or kTravOccluded (any-hit only).
+struct TravRay {
+    V3 o, d;
+    float ix, iy, iz, ox, oy, oz;
+    int node;
+};
+enum { kTravRunning = 0, kTravDone = 1, kTravOccluded = 2 };
+
+// Returns false when the ray cannot hit anything (zero direction).
+__device__ __forceinline__ bool trav_begin(TravRay& r, V3 o, V3 d) {
+    r.o = o;
+    r.d = d;
+    r.ix = 1.f / d.x;
+    r.iy = 1.f / d.y;
+    r.iz = 1.f / d.z;
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    r.node = 0;
+    return !(d.x == 0.f && d.y == 0.f && d.z == 0.f);
+}
+
+template <bool ANY, bool COUNT, class StackType>
+__device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris, TravRay& r, float tmin, HitRec& best,
+                                         StackType& st, Counts& cnt) {
+    const int node = r.node;
+    if (node >= 0) {
+        if (COUNT) cnt.nodes++;
+        r.node = visit_node(nodes, node, r.ix, r.iy, r.iz, r.ox, r.oy, r.oz, tmin, best.t, st);
+        if (r.node != kNoChild) return kTravRunning;
+    } else {
+        const uint32_t w = uint32_t(node);
+        const uint32_t first = w & 0x0FFFFFFFu;
+        const uint32_t count = ((w >> 28) & 7u) + 1u;
+        for (uint32_t k = 0; k < count; ++k) {
+            const size_t tb = size_t(first + k) * 3;
+            float4 a = gload(tris, tb), b = gload(tris, tb + 1), c = gload(tris, tb + 2);
+            if (COUNT) cnt.tris++;
+            float t, u, v;
+            if (!mt_intersect(a, b, c, r.o, r.d, t, u, v)) continue;
+            uint32_t id = __float_as_uint(a.w);
+            if (!(t > tmin)) continue;
+            if (t < best.t || (t == best.t && id < best.id)) {
+                if (ANY) {
+                    st.sp = 0;
+                    return kTravOccluded;
+                }
+                best.t = t;
+                best.u = u;
+                best.v = v;
+                best.rec = first + k;
+                best.id = id;
+                best.mat = __float_as_uint(b.w);
+            }
+        }
+    }
+    if (st.sp == 0) return kTravDone;
+    r.node = st.pop();
+    return kTravRunning;
 }
 
 // Interpolated shading normal (Triangle::getNormal, geometry.cu:67-71); shading

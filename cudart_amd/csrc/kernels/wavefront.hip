@@ -155,6 +155,54 @@ __device__ __forceinline__ SpreadFetch make_fetch(uint32_t* ctr, uint32_t n) {
     return SpreadFetch{ctr, wave % kSpread, (n + 63) / 64};
 }
 
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Hands single items of [0, n) to individual lanes: the wave holds one 64-item
+// batch of a SpreadFetch at a time (one atomic per 64 items, the next one already
+// in flight) and gives its items to whichever lanes ask, so a lane whose ray has
+// finished takes a new one while the others keep traversing. Every issued atomic
+// is resolved before more() turns false. Calls must be wave-uniform.
+struct LaneFeeder {
+    SpreadFetch fetch;
+    uint32_t n, base, used, pending;
+    __device__ __forceinline__ void start(uint32_t* ctr, uint32_t count) {
+        fetch = make_fetch(ctr, count);
+        n = count;
+        base = n ? fetch.resolve(fetch.issue()) : kNone;
+        used = 0;
+        pending = base != kNone ? fetch.issue() : kNone;
+    }
+    __device__ __forceinline__ bool more() const { return base != kNone; }
+    // Lanes with `want` get the next item, or kNone once every item is handed out.
+    __device__ __forceinline__ uint32_t take(bool want) {
+        const uint32_t lane = threadIdx.x & 63;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t item = kNone;
+        unsigned long long m = __ballot(want);
+        while (m != 0ull && base != kNone) {
+            const uint32_t size = min(64u, n - base);
+            const uint32_t avail = size - used;
+            const uint32_t rank = uint32_t(__popcll(m & below));
+            if (((m >> lane) & 1ull) && rank < avail) item = base + used + rank;
+            used += min(uint32_t(__popcll(m)), avail);
+            if (used == size) {
+                base = fetch.resolve(pending);
+                used = 0;
+                pending = base != kNone ? fetch.issue() : kNone;
+            }
+            m = __ballot(want && item == kNone);
+        }
+        return item;
+    }
+};
+
+// Idle lanes of a wave that trigger a refill (the rest keep traversing meanwhile).
+// kRefill <= 64, so a wave with no active lane always refills or, with no work
+// left, leaves the loop.
+constexpr uint32_t kRefill = 16;
+
+__device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__popcll(__ballot(!active))); }
+
 // Adds the wave's counts to the counter slots; ANYHIT also books nodes and
 // triangles as the shadow (any-hit) kernel's share.
 template <bool ANYHIT>
@@ -230,7 +278,9 @@ __device__ __forceinline__ void with_job(bool valid, uint32_t job, F&& f) {
     }
 }
 
-template <bool COUNT>
+// ONE_SCENE: every job of the batch traverses the same scene, so the node and
+// triangle arrays are wave-uniform (scalar registers).
+template <bool COUNT, bool ONE_SCENE>
 __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
     __shared__ int s_stack[kStackLdsQueue * kBlock];
     const uint32_t q = it & 1u;
@@ -241,39 +291,53 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
     }
     const uint32_t lane = threadIdx.x;
     StackT<kStackLdsQueue> st;
-    st.lds = s_stack + lane;
-    st.spill_stride = a.spill_stride;
-    st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
-    st.sp = 0;
+    st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
-    SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it) * kSpread], n);
-    uint32_t base = n ? fetch.resolve(fetch.issue()) : 0xFFFFFFFFu;
-    while (base < n) {
-        const uint32_t pending = fetch.issue();
-        const uint32_t i = base + lane;
-        const bool valid = i < n;
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
-        if (valid) {
-            o = a.qa[q][i];
-            d = a.qb[q][i];
-        }
-        const uint32_t job = __float_as_uint(o.w) >> 24;
-        with_job(valid, job, [&](uint32_t jb) {
-            const SceneArgs& sc = a.jobs[jb].sc;
-            if (COUNT) {
-                if (it == 0) cnt.primary++; else cnt.extension++;
+    LaneFeeder feed;
+    feed.start(&a.ctr[kFetch + (3 * it) * kSpread], n);
+    bool active = false;
+    uint32_t i = 0;
+    TravRay ray;
+    HitRec h;
+    const float4* nodes = ONE_SCENE ? a.jobs[0].sc.nodes : nullptr;
+    const float4* tris = ONE_SCENE ? a.jobs[0].sc.tris : nullptr;
+    while (true) {
+        const uint32_t idle = idle_lanes(active);
+        if (idle >= kRefill) {
+            if (!feed.more()) {
+                if (idle == 64) break;
+            } else if (const uint32_t got = feed.take(!active); got != kNone) {
+                i = got;
+                const float4 o = a.qa[q][i], d = a.qb[q][i];
+                if (!ONE_SCENE) {
+                    const SceneArgs& sc = a.jobs[__float_as_uint(o.w) >> 24].sc;
+                    nodes = sc.nodes;
+                    tris = sc.tris;
+                }
+                if (COUNT) {
+                    if (it == 0) cnt.primary++; else cnt.extension++;
+                }
+                h.t = kFltMax;
+                h.id = 0;
+                h.rec = 0xFFFFFFFFu;
+                h.mat = 0;
+                h.u = h.v = 0.f;
+                st.sp = 0;
+                active = trav_begin(ray, v3(o), v3(d));
+                if (!active) {  // zero direction: no triangle can be hit
+                    a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(0xFFFFFFFFu));
+                    a.hmat[i] = 0;
+                }
             }
-            HitRec h;
-            h.t = kFltMax;
-            h.id = 0;
-            h.rec = 0xFFFFFFFFu;
-            h.mat = 0;
-            h.u = h.v = 0.f;
-            bool hit = traverse<false, COUNT>(sc, v3(o), v3(d), kEps, h, st, cnt);
-            a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.rec : 0xFFFFFFFFu));
-            a.hmat[i] = h.mat;
-        });
-        base = fetch.resolve(pending);
+        }
+        // One node or leaf per active lane; a refill is considered every step.
+        if (active) {
+            if (trav_step<false, COUNT>(nodes, tris, ray, kEps, h, st, cnt) != kTravRunning) {
+                a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.rec));
+                a.hmat[i] = h.mat;
+                active = false;
+            }
+        }
     }
     if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
@@ -380,59 +444,73 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
     if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool ONE_SCENE>
 __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     __shared__ int s_stack[kStackLdsQueue * kBlock];
     const uint32_t q1 = (it & 1u) ^ 1u;  // the queue shade(it) filled
     const uint32_t n = a.ctr[kShadow];
     const uint32_t lane = threadIdx.x;
     StackT<kStackLdsQueue> st;
-    st.lds = s_stack + lane;
-    st.spill_stride = a.spill_stride;
-    st.spill = a.spill + size_t(blockIdx.x) * kBlock + lane;
-    st.sp = 0;
+    st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
-    SpreadFetch fetch = make_fetch(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
-    uint32_t base = n ? fetch.resolve(fetch.issue()) : 0xFFFFFFFFu;
-    while (base < n) {
-        const uint32_t pending = fetch.issue();
-        const uint32_t i = base + lane;
-        const bool valid = i < n;
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f), f = e, g = e, hx = e;
-        if (valid) {
-            e = a.se[i];
-            f = a.sf[i];
-            g = a.sg[i];
-            hx = a.sh[i];
-        }
-        with_job(valid, __float_as_uint(hx.z), [&](uint32_t jb) {
-            const SceneArgs& sc = a.jobs[jb].sc;
-            NeeSample ns;
-            ns.position = v3(e);
-            ns.normal = v3(f);
-            ns.mask = v3(g);
-            ns.li = __float_as_uint(g.w);
-            ns.lu = f.w;
-            ns.lv = hx.x;
-            ShadowReq sr;
-            sr.direct = v3(0.f, 0.f, 0.f);
-            bool visible = false;
-            if (nee_prepare<COUNT>(sc, ns, sr, cnt)) {
-                HitRec occ;
+    LaneFeeder feed;
+    feed.start(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
+    bool active = false;
+    TravRay ray;
+    HitRec occ;
+    const float4* nodes = ONE_SCENE ? a.jobs[0].sc.nodes : nullptr;
+    const float4* tris = ONE_SCENE ? a.jobs[0].sc.tris : nullptr;
+    float4* dst = nullptr;  // the path's colour: next-queue entry or result slot
+    V3 mask = v3(0.f, 0.f, 0.f), direct = mask;
+    while (true) {
+        const uint32_t idle = idle_lanes(active);
+        if (idle >= kRefill) {
+            if (!feed.more()) {
+                if (idle == 64) break;
+            } else if (const uint32_t got = feed.take(!active); got != kNone) {
+                const float4 e = a.se[got], f = a.sf[got], g = a.sg[got], hx = a.sh[got];
+                const SceneArgs& sc = a.jobs[__float_as_uint(hx.z)].sc;
+                NeeSample ns;
+                ns.position = v3(e);
+                ns.normal = v3(f);
+                ns.mask = v3(g);
+                ns.li = __float_as_uint(g.w);
+                ns.lu = f.w;
+                ns.lv = hx.x;
+                ShadowReq sr;
+                sr.direct = v3(0.f, 0.f, 0.f);
+                const bool reach = nee_prepare<COUNT>(sc, ns, sr, cnt);
+                const uint32_t target = __float_as_uint(e.w);
+                dst = __float_as_uint(hx.y) ? a.result + target : a.qd[q1] + target;
+                mask = ns.mask;
+                direct = sr.direct;
+                if (!ONE_SCENE) {
+                    nodes = sc.nodes;
+                    tris = sc.tris;
+                }
                 occ.t = sr.tL;
                 occ.id = sr.lid;
                 occ.rec = 0;
                 occ.mat = 0;
                 occ.u = occ.v = 0.f;
-                visible = !traverse<true, COUNT>(sc, sr.so, sr.sd, kEps, occ, st, cnt);
+                st.sp = 0;
+                active = reach && trav_begin(ray, sr.so, sr.sd);
+                if (!active) {  // the light sample cannot be seen: add mask * 0
+                    const float4 c = *dst;
+                    const V3 s = add_direct(v3(c), mask, direct, false);
+                    *dst = make_float4(s.x, s.y, s.z, c.w);
+                }
             }
-            const uint32_t target = __float_as_uint(e.w);
-            float4* dst = __float_as_uint(hx.y) ? a.result + target : a.qd[q1] + target;
-            const float4 c = *dst;
-            const V3 s = add_direct(v3(c), ns.mask, sr.direct, visible);
-            *dst = make_float4(s.x, s.y, s.z, c.w);
-        });
-        base = fetch.resolve(pending);
+        }
+        if (active) {
+            const int r = trav_step<true, COUNT>(nodes, tris, ray, kEps, occ, st, cnt);
+            if (r != kTravRunning) {
+                const float4 c = *dst;
+                const V3 s = add_direct(v3(c), mask, direct, r == kTravDone);
+                *dst = make_float4(s.x, s.y, s.z, c.w);
+                active = false;
+            }
+        }
     }
     if (COUNT) flush_stats<true>(a.ctr, cnt);
 }
@@ -596,28 +674,36 @@ struct RunStats {
 // Launches one chunk's kernels on `stream`: setup, generate, then per iteration
 // extend, shade and shadow, then resolve and advance. With `ev` (4 per iteration)
 // the extend / shade / shadow launches are bracketed by events.
-template <bool COUNT>
+template <bool COUNT, bool ONE_SCENE>
 void launch_chunk(const WfArgs& a, hipStream_t stream, uint32_t max_depth, int pblocks, int gblocks, int sblocks,
                   hipEvent_t* ev) {
     hipLaunchKernelGGL(dev::wf_setup, dim3(1), dim3(64), 0, stream, a);
     hipLaunchKernelGGL(dev::wf_generate, dim3(gblocks), dim3(256), 0, stream, a);
     for (uint32_t it = 0; it < max_depth; ++it) {
         if (ev) (void)hipEventRecord(ev[4 * it], stream);
-        hipLaunchKernelGGL(dev::wf_extend<COUNT>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        hipLaunchKernelGGL((dev::wf_extend<COUNT, ONE_SCENE>), dim3(pblocks), dim3(kBlock), 0, stream, a, it);
         if (ev) (void)hipEventRecord(ev[4 * it + 1], stream);
         hipLaunchKernelGGL(dev::wf_shade<COUNT>, dim3(sblocks), dim3(kShadeBlock), 0, stream, a, it);
         if (ev) (void)hipEventRecord(ev[4 * it + 2], stream);
-        hipLaunchKernelGGL(dev::wf_shadow<COUNT>, dim3(pblocks), dim3(kBlock), 0, stream, a, it);
+        hipLaunchKernelGGL((dev::wf_shadow<COUNT, ONE_SCENE>), dim3(pblocks), dim3(kBlock), 0, stream, a, it);
         if (ev) (void)hipEventRecord(ev[4 * it + 3], stream);
     }
     hipLaunchKernelGGL(dev::wf_resolve, dim3(gblocks), dim3(256), 0, stream, a);
     hipLaunchKernelGGL(dev::wf_advance, dim3(1), dim3(64), 0, stream, a);
 }
 
+void launch_chunk_any(bool count, bool one_scene, const WfArgs& a, hipStream_t stream, uint32_t max_depth,
+                      int pblocks, int gblocks, int sblocks, hipEvent_t* ev) {
+    if (count && one_scene) launch_chunk<true, true>(a, stream, max_depth, pblocks, gblocks, sblocks, ev);
+    else if (count) launch_chunk<true, false>(a, stream, max_depth, pblocks, gblocks, sblocks, ev);
+    else if (one_scene) launch_chunk<false, true>(a, stream, max_depth, pblocks, gblocks, sblocks, ev);
+    else launch_chunk<false, false>(a, stream, max_depth, pblocks, gblocks, sblocks, ev);
+}
+
 // Profile mode: every chunk launched directly, each extend / shade / shadow launch
 // timed with events; per-kernel device time summed into rs.
 int run_profiled(const WfArgs& a, hipStream_t stream, size_t nchunks, uint32_t max_depth, int pblocks, int gblocks,
-                 int sblocks, bool count, RunStats* rs) {
+                 int sblocks, bool count, bool one_scene, RunStats* rs) {
     std::vector<hipEvent_t> ev(4 * max_depth, nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = PRT_OK;
@@ -631,8 +717,7 @@ int run_profiled(const WfArgs& a, hipStream_t stream, size_t nchunks, uint32_t m
     for (size_t c = 0; c < nchunks && e == hipSuccess; ++c) {
         e = hipEventRecord(e0, stream);
         if (e != hipSuccess) break;
-        if (count) launch_chunk<true>(a, stream, max_depth, pblocks, gblocks, sblocks, ev.data());
-        else launch_chunk<false>(a, stream, max_depth, pblocks, gblocks, sblocks, ev.data());
+        launch_chunk_any(count, one_scene, a, stream, max_depth, pblocks, gblocks, sblocks, ev.data());
         e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(e1, stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -675,8 +760,13 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     if (chunks.empty()) return PRT_OK;
     uint32_t need = 0;
     for (auto& c : chunks) need = std::max(need, c.paths);
-    const void* kext = count ? reinterpret_cast<const void*>(dev::wf_extend<true>)
-                             : reinterpret_cast<const void*>(dev::wf_extend<false>);
+    bool one_scene = true;
+    for (const WfJob& j : jobs)
+        one_scene = one_scene && j.sc.nodes == jobs[0].sc.nodes && j.sc.tris == jobs[0].sc.tris;
+    const void* kext = count ? (one_scene ? reinterpret_cast<const void*>(dev::wf_extend<true, true>)
+                                          : reinterpret_cast<const void*>(dev::wf_extend<true, false>))
+                             : (one_scene ? reinterpret_cast<const void*>(dev::wf_extend<false, true>)
+                                          : reinterpret_cast<const void*>(dev::wf_extend<false, false>));
     const int pblocks = persistent_blocks(kext, blocks_per_cu, device);
     int rc = ensure_state(w, device, need, uint32_t(pblocks) * kBlock);
     if (rc != PRT_OK) return rc;
@@ -701,7 +791,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     const int sblocks = prop.multiProcessorCount * 2;  // 2 x 8 waves of shade per compute unit
 
     if (profile) {
-        rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, rs);
+        rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, one_scene, rs);
         if (rc == PRT_OK && rs) WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
         return rc;
     }
@@ -709,8 +799,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WF_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    if (count) launch_chunk<true>(a, stream, max_depth, pblocks, gblocks, sblocks, nullptr);
-    else launch_chunk<false>(a, stream, max_depth, pblocks, gblocks, sblocks, nullptr);
+    launch_chunk_any(count, one_scene, a, stream, max_depth, pblocks, gblocks, sblocks, nullptr);
     hipError_t e = hipStreamEndCapture(stream, &graph);
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

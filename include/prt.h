@@ -142,7 +142,7 @@ typedef struct prt_stats {
 
 /* BVH statistics of a built scene. */
 typedef struct prt_bvh_info {
-    uint32_t num_nodes;        /* child-pair nodes */
+    uint32_t num_nodes;        /* four-wide nodes (128 B each) */
     uint32_t num_leaves;
     uint32_t max_depth;
     uint32_t max_leaf_size;
@@ -150,6 +150,7 @@ typedef struct prt_bvh_info {
     uint64_t triangle_bytes;   /* intersection records */
     uint64_t shading_bytes;    /* shading records */
     float sah_cost;
+    uint32_t node_read_bytes;  /* bytes one node visit reads (the four boxes and child words) */
 } prt_bvh_info;
 
 typedef struct prt_mesh prt_mesh;         /* owned host scene description (loader / generator output) */
@@ -214,7 +215,9 @@ int prt_env_load(const char *env_path, char *meshes, char *cameras, char *textur
 int prt_scene_create(const prt_scene_desc *desc, prt_scene **out);
 int prt_scene_bvh_info(const prt_scene *scene, prt_bvh_info *info);
 /* Copies the built layout out, for host-side validation:
- * nodes: num_nodes*16 floats (64 B each); tris: n*12 floats (48 B each);
+ * nodes: num_nodes*32 floats (128 B each: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
+ * of the four children, then four child words: >= 0 inner node, 0x7FFFFFFF empty slot,
+ * negative = leaf 0x80000000 | (count-1) << 28 | first record); tris: n*12 floats (48 B each);
  * tri_order: n uint32 (original triangle id of each record). Any pointer may be NULL. */
 int prt_scene_export_bvh(const prt_scene *scene, float *nodes, float *tris, uint32_t *tri_order);
 int prt_scene_num_lights(const prt_scene *scene, uint32_t *num_lights, float *lights_area);

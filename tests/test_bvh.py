@@ -1,4 +1,4 @@
-"""BVH invariants (product child-pair layout and the oracle's reference-shaped
+"""BVH invariants (product four-wide layout and the oracle's reference-shaped
 tree) and closest-hit equality with an O(N) brute-force scan."""
 import numpy as np
 import pytest
@@ -26,7 +26,7 @@ def _check_product_layout(mesh):
     np.testing.assert_array_equal(recs[:, 8:11], p[:, 2] - p[:, 0])
     assert np.array_equal(recs[:, 3].view(np.uint32), order)
     assert np.array_equal(recs[:, 7].view(np.uint32), a.material_ids[order])
-    words = nodes[:, 12:14].view(np.int32)
+    words = nodes[:, 24:28].view(np.int32)
     lo, hi = _tri_bounds(a.positions)
     covered = np.zeros(n, np.int32)
     seen_nodes = np.zeros(len(nodes), np.int32)
@@ -37,14 +37,21 @@ def _check_product_layout(mesh):
         seen_nodes[k] += 1
         depth_max = max(depth_max, depth)
         nd = nodes[k]
-        for c in range(2):
-            clo = np.array([nd[c * 4 + 0], nd[c * 4 + 2], nd[8 + c * 2]])
-            chi = np.array([nd[c * 4 + 1], nd[c * 4 + 3], nd[8 + c * 2 + 1]])
+        assert (nd[28:32] == 0).all()
+        used = 0
+        for c in range(4):
+            clo = np.array([nd[c], nd[8 + c], nd[16 + c]])
+            chi = np.array([nd[4 + c], nd[12 + c], nd[20 + c]])
             w = int(words[k, c])
-            if clo[0] > chi[0]:  # inverted box: the empty partner of a one-leaf root
+            if w == 0x7FFFFFFF:  # empty slot: zero box, only after the used ones
+                assert (clo == 0).all() and (chi == 0).all()
                 continue
+            assert used == c, "empty slots come last"
+            used += 1
+            assert (clo <= chi).all()
             assert (clo >= plo).all() and (chi <= phi).all(), "child box outside its parent's"
             if w >= 0:
+                assert w < len(nodes)
                 stack.append((w, clo, chi, depth + 1))
             else:
                 u = w & 0xFFFFFFFF
@@ -54,6 +61,7 @@ def _check_product_layout(mesh):
                 assert (lo[ids] >= clo).all() and (hi[ids] <= chi).all(), "leaf box does not contain a triangle"
                 assert (lo[ids] > clo).all() and (hi[ids] < chi).all(), "leaf box is not padded"
                 covered[first:first + count] += 1
+        assert used >= 1 and (used >= 2 or len(nodes) == 1), "an inner node holds at least two children"
     assert (covered == 1).all(), "every record in exactly one leaf"
     assert (seen_nodes == 1).all(), "every node reached once"
     assert depth_max == info.max_depth
