@@ -166,11 +166,11 @@ class Scene:
 
     def export_bvh(self):
         info = self.bvh_info()
-        nodes = np.zeros(info.num_nodes * 32, np.float32)
+        nodes = np.zeros(info.num_nodes * 16, np.float32)
         tris = np.zeros(self.num_triangles * 12, np.float32)
         order = np.zeros(self.num_triangles, np.uint32)
         check(lib().prt_scene_export_bvh(self._h, _capi.fptr(nodes), _capi.fptr(tris), _capi.uptr(order)))
-        return nodes.reshape(-1, 32), tris.reshape(-1, 12), order
+        return nodes.reshape(-1, 16), tris.reshape(-1, 12), order
 
     def lights(self):
         n = C.c_uint32()

@@ -6,9 +6,11 @@ fallback: if the library is missing, loading raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libprt.so"
+# PRT_LIBRARY selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = Path(os.environ.get("PRT_LIBRARY") or Path(__file__).resolve().parent / "lib" / "libprt.so")
 
 PRT_OK = 0
 ERRORS = {

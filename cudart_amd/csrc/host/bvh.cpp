@@ -167,6 +167,46 @@ void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth
     }
 }
 
+// Quantizes the children's boxes (lo[3], hi[3]; null = empty slot) against their
+// union. Per axis the step is the smallest power of two with 255 steps covering the
+// extent; q_lo = floor((lo - origin) / step) and q_hi = ceil((hi - origin) / step),
+// computed exactly in double, so fmaf(q, step, origin) (one rounding, monotonic)
+// lands at or beyond the exact plane.
+void write_node(float* o, const float* const boxes[4], const int32_t words[4]) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(o);
+    std::memset(o, 0, kNodeFloats * sizeof(float));
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int c = 0; c < 4; ++c)
+        if (boxes[c])
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], boxes[c][k]);
+                hi[k] = std::max(hi[k], boxes[c][3 + k]);
+            }
+    uint32_t exps = 0;
+    double step[3];
+    for (int k = 0; k < 3; ++k) {
+        const double ext = double(hi[k]) - double(lo[k]);
+        int e = -126;
+        while (e < 127 && std::ldexp(255.0, e) < ext) ++e;
+        step[k] = std::ldexp(1.0, e);
+        exps |= uint32_t(e + 127) << (8 * k);
+        std::memcpy(&w[k], &lo[k], 4);
+    }
+    w[3] = exps;
+    for (int c = 0; c < 4; ++c) std::memcpy(&w[4 + c], &words[c], 4);
+    for (int c = 0; c < 4; ++c) {
+        if (!boxes[c]) continue;
+        for (int k = 0; k < 3; ++k) {
+            const double ql = std::floor((double(boxes[c][k]) - double(lo[k])) / step[k]);
+            const double qh = std::ceil((double(boxes[c][3 + k]) - double(lo[k])) / step[k]);
+            const uint32_t a = uint32_t(std::max(0.0, std::min(255.0, ql)));
+            const uint32_t b = uint32_t(std::max(0.0, std::min(255.0, qh)));
+            w[8 + 2 * k] |= a << (8 * c);
+            w[9 + 2 * k] |= b << (8 * c);
+        }
+    }
+}
+
 // Writes the four-wide layout depth first from the binary build tree.
 struct Emitter {
     const BNode* bn;
@@ -216,15 +256,9 @@ struct Emitter {
         const int n = gather(b, slot);
         int32_t words[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
         for (int c = 0; c < n; ++c) words[c] = child_word(slot[c], depth);
-        float* o = &out[size_t(k) * kNodeFloats];
-        for (int c = 0; c < 4; ++c) {
-            const float* bx = c < n ? pbox[slot[c]] : nullptr;  // lo[3], hi[3]
-            for (int axis = 0; axis < 3; ++axis) {
-                o[axis * 8 + c] = bx ? bx[axis] : 0.f;
-                o[axis * 8 + 4 + c] = bx ? bx[3 + axis] : 0.f;
-            }
-            std::memcpy(&o[24 + c], &words[c], 4);
-        }
+        const float* boxes[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int c = 0; c < n; ++c) boxes[c] = pbox[slot[c]];
+        write_node(&out[size_t(k) * kNodeFloats], boxes, words);
     }
 };
 
@@ -295,13 +329,9 @@ void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout&
     if (inner == 0) {
         // Whole scene in one leaf: the root holds it in slot 0, the other slots empty.
         out.nodes.assign(kNodeFloats, 0.f);
-        float* o = out.nodes.data();
-        for (int axis = 0; axis < 3; ++axis) {
-            o[axis * 8] = pb[axis];
-            o[axis * 8 + 4] = pb[3 + axis];
-        }
+        const float* boxes[4] = {&pb[0], nullptr, nullptr, nullptr};
         const int32_t words[4] = {leaf_word(0, n), kEmptyChild, kEmptyChild, kEmptyChild};
-        std::memcpy(&o[24], words, sizeof words);
+        write_node(out.nodes.data(), boxes, words);
         out.num_nodes = 1;
         out.num_leaves = 1;
         out.max_leaf = n;

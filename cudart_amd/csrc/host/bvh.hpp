@@ -9,19 +9,21 @@
 
 namespace prt {
 
-// Four-wide node, 128 bytes = 32 floats (one 128-byte line; a visit reads the
-// first 112 bytes), boxes of the four children as structure of arrays:
-//   [0..3]   lo.x of children 0..3     [4..7]   hi.x
-//   [8..11]  lo.y                      [12..15] hi.y
-//   [16..19] lo.z                      [20..23] hi.z
-//   [24..27] child words (int32 bit patterns)      [28..31] 0
-// A child word >= 0 is an inner node index, kEmptyChild an unused slot (its box
-// is zero). A negative word is a leaf:
-//   bit 31 set, bits 28..30 = count-1 (1..8 triangles), bits 0..27 = first record.
+// Four-wide node with quantized child boxes, 64 bytes = 16 words (one cache line):
+//   [0..2]   origin: the node box's lower corner (float)
+//   [3]      per axis the biased float exponent of the step 2^e (byte k = axis k)
+//   [4..7]   child words (int32)
+//   [8..13]  qlo.x qhi.x qlo.y qhi.y qlo.z qhi.z: 4 bytes each, byte c = child c
+//   [14..15] 0
+// Child c spans fmaf(q, 2^e, origin) per plane: q rounded outward, so each decoded
+// box contains the child's exact box (culling stays conservative).
+// A child word >= 0 is an inner node index, kEmptyChild an unused slot. A negative
+// word is a leaf: bit 31 set, bits 28..30 = count-1 (1..8 triangles), bits 0..27 =
+// first record.
 // Collapsed from a binary SAH tree: a node takes its two children and keeps
 // opening the inner child with the largest box area until it has four.
-constexpr int kNodeFloats = 32;
-constexpr uint32_t kNodeReadBytes = 112;
+constexpr int kNodeFloats = 16;
+constexpr uint32_t kNodeReadBytes = 64;
 constexpr int32_t kEmptyChild = 0x7FFFFFFF;
 constexpr int kMaxLeaf = 8;
 constexpr uint32_t kMaxRecords = 1u << 28;

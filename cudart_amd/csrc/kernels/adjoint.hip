@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
         const int y = int(tile / a.tiles_x) * 8 + (lane >> 3);
         if (x < int(a.width) && y < int(a.height)) {
             Stack st;
-            st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
+            st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
             Counts cnt = {0, 0, 0, 0, 0, 0};
             const size_t idx = (size_t(y) * a.width + size_t(x)) * 3;
             const D3 delta{a.norm * double(a.grad[idx]), a.norm * double(a.grad[idx + 1]),

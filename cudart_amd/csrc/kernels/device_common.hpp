@@ -149,26 +149,29 @@ using glob_int = __attribute__((address_space(1))) int;
 template <int LDS>
 struct StackT {
     static constexpr int kLds = LDS;
-    lds_int* lds;          // this lane's column: lds[k * kBlock]
-    glob_int* spill;       // this thread's column in the spill area
+    lds_int* lds;          // the workgroup's LDS stack block; lane l uses lds[k * kBlock + l]
+    glob_int* spill;       // the workgroup's spill block; lane l uses spill[k * stride + l]
     uint32_t spill_stride; // threads sharing the spill area
     int sp;
-    __device__ __forceinline__ void bind(int* lds_col, int* spill_col, uint32_t stride) {
-        lds = (lds_int*)lds_col;
-        spill = (glob_int*)spill_col;
+    // Both bases are workgroup-uniform; the lane offset is added per access.
+    __device__ __forceinline__ void bind(int* lds_block, int* spill_block, uint32_t stride) {
+        lds = (lds_int*)lds_block;
+        spill = (glob_int*)spill_block;
         spill_stride = stride;
         sp = 0;
     }
     __device__ __forceinline__ void push(int v) {
-        if (sp < LDS) lds[sp * kBlock] = v;
-        else spill[size_t(sp - LDS) * spill_stride] = v;
+        const uint32_t lane = threadIdx.x;
+        if (sp < LDS) lds[sp * kBlock + lane] = v;
+        else spill[size_t(sp - LDS) * spill_stride + lane] = v;
         ++sp;
     }
     __device__ __forceinline__ int pop() {
+        const uint32_t lane = threadIdx.x;
         --sp;
         int v;
-        if (sp < LDS) v = lds[sp * kBlock];
-        else v = spill[size_t(sp - LDS) * spill_stride];
+        if (sp < LDS) v = lds[sp * kBlock + lane];
+        else v = spill[size_t(sp - LDS) * spill_stride + lane];
         return v;
     }
 };
@@ -198,9 +201,11 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
     return t > kEps;
 }
 
-// One four-wide node: slab tests of the four child boxes against [tmin, tfar],
-// then the children hit, nearest first. Pushes the farther ones (farthest first,
-// so the nearest of them is popped next) and returns the nearest, or kNoChild.
+// One four-wide node (64 bytes, quantized child boxes): decodes each child plane as
+// fmaf(q, 2^e, origin) (exactly as the builder bounded it), slab-tests the four
+// boxes against [tmin, tfar], and orders the children hit nearest first. Pushes the
+// farther ones (farthest first, so the nearest of them is popped next) and returns
+// the nearest, or kNoChild.
 constexpr int kNoChild = 0x7FFFFFFF;  // also the word of an empty slot
 __device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
     const bool sw = kb < ka;
@@ -211,28 +216,35 @@ __device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
     wb = sw ? wa : wb;
     wa = w;
 }
+__device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xFFu); }
 template <class StackType>
 __device__ __forceinline__ int visit_node(const float4* nodes, int node, float ix, float iy, float iz, float ox,
                                           float oy, float oz, float tmin, float tfar, StackType& st) {
-    const size_t nb = size_t(node) * 8;
-    const float4 lx = gload(nodes, nb), hx = gload(nodes, nb + 1), ly = gload(nodes, nb + 2),
-                 hy = gload(nodes, nb + 3), lz = gload(nodes, nb + 4), hz = gload(nodes, nb + 5);
-    const int4 w = gload(reinterpret_cast<const int4*>(nodes), nb + 6);
+    const size_t nb = size_t(node) * 4;
+    const uint4* un = reinterpret_cast<const uint4*>(nodes);
+    const float4 org = gload(nodes, nb);
+    const uint4 w = gload(un, nb + 1), qxy = gload(un, nb + 2), qz = gload(un, nb + 3);
+    const uint32_t ex = __float_as_uint(org.w);
+    const float sx = __uint_as_float((ex & 0xFFu) << 23), sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
     float key[4];
-    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-    int word[4] = {w.x, w.y, w.z, w.w};
+    int word[4] = {int(w.x), int(w.y), int(w.z), int(w.w)};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float a0 = __builtin_fmaf(lxa[c], ix, -ox), b0 = __builtin_fmaf(hxa[c], ix, -ox);
-        const float a1 = __builtin_fmaf(lya[c], iy, -oy), b1 = __builtin_fmaf(hya[c], iy, -oy);
-        const float a2 = __builtin_fmaf(lza[c], iz, -oz), b2 = __builtin_fmaf(hza[c], iz, -oz);
+        const float lx = __builtin_fmaf(qbyte(qxy.x, c), sx, org.x), hx = __builtin_fmaf(qbyte(qxy.y, c), sx, org.x);
+        const float ly = __builtin_fmaf(qbyte(qxy.z, c), sy, org.y), hy = __builtin_fmaf(qbyte(qxy.w, c), sy, org.y);
+        const float lz = __builtin_fmaf(qbyte(qz.x, c), sz, org.z), hz = __builtin_fmaf(qbyte(qz.y, c), sz, org.z);
+        const float a0 = __builtin_fmaf(lx, ix, -ox), b0 = __builtin_fmaf(hx, ix, -ox);
+        const float a1 = __builtin_fmaf(ly, iy, -oy), b1 = __builtin_fmaf(hy, iy, -oy);
+        const float a2 = __builtin_fmaf(lz, iz, -oz), b2 = __builtin_fmaf(hz, iz, -oz);
         const float tn = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
         const float tf = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tfar));
         const bool hit = tn <= tf && word[c] != kNoChild;
         key[c] = hit ? tn : __builtin_inff();
         word[c] = hit ? word[c] : kNoChild;
+        // One child at a time: interleaving the four decodes would hold all 24
+        // decoded planes live (register pressure, hence occupancy).
+        __builtin_amdgcn_sched_barrier(0);
     }
     cswap(key[0], word[0], key[1], word[1]);
     cswap(key[2], word[2], key[3], word[3]);
@@ -304,7 +316,7 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
 // occluder) or kTravOccluded (any-hit only).
 struct TravRay {
     V3 o, d;
-    float ix, iy, iz, ox, oy, oz;
+    float ix, iy, iz;  // o * i is recomputed per node (the same fp32 products as traverse's)
     int node;
 };
 enum { kTravRunning = 0, kTravDone = 1, kTravOccluded = 2 };
@@ -316,20 +328,19 @@ __device__ __forceinline__ bool trav_begin(TravRay& r, V3 o, V3 d) {
     r.ix = 1.f / d.x;
     r.iy = 1.f / d.y;
     r.iz = 1.f / d.z;
-    r.ox = o.x * r.ix;
-    r.oy = o.y * r.iy;
-    r.oz = o.z * r.iz;
     r.node = 0;
     return !(d.x == 0.f && d.y == 0.f && d.z == 0.f);
 }
 
-template <bool ANY, bool COUNT, class StackType>
+// LEAN keeps only t, rec and id of the best hit (u, v and the material are not
+// written; hit_details() recomputes them from the record afterwards).
+template <bool ANY, bool COUNT, bool LEAN, class StackType>
 __device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris, TravRay& r, float tmin, HitRec& best,
                                          StackType& st, Counts& cnt) {
     const int node = r.node;
     if (node >= 0) {
         if (COUNT) cnt.nodes++;
-        r.node = visit_node(nodes, node, r.ix, r.iy, r.iz, r.ox, r.oy, r.oz, tmin, best.t, st);
+        r.node = visit_node(nodes, node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz, tmin, best.t, st);
         if (r.node != kNoChild) return kTravRunning;
     } else {
         const uint32_t w = uint32_t(node);
@@ -349,17 +360,31 @@ __device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris
                     return kTravOccluded;
                 }
                 best.t = t;
-                best.u = u;
-                best.v = v;
                 best.rec = first + k;
                 best.id = id;
-                best.mat = __float_as_uint(b.w);
+                if (!LEAN) {
+                    best.u = u;
+                    best.v = v;
+                    best.mat = __float_as_uint(b.w);
+                }
             }
         }
     }
     if (st.sp == 0) return kTravDone;
     r.node = st.pop();
     return kTravRunning;
+}
+
+// u, v and material of best.rec, by the intersection test that selected it
+// (identical inputs, so identical results).
+__device__ __forceinline__ void hit_details(const float4* tris, const TravRay& r, HitRec& best) {
+    const size_t tb = size_t(best.rec) * 3;
+    const float4 a = gload(tris, tb), b = gload(tris, tb + 1), c = gload(tris, tb + 2);
+    float t, u, v;
+    mt_intersect(a, b, c, r.o, r.d, t, u, v);
+    best.u = u;
+    best.v = v;
+    best.mat = __float_as_uint(b.w);
 }
 
 // Interpolated shading normal (Triangle::getNormal, geometry.cu:67-71); shading

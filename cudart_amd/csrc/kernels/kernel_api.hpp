@@ -10,7 +10,10 @@ namespace prt {
 
 constexpr int kBlock = 64;       // one wave64 per workgroup: an 8x8 pixel tile
 constexpr int kStackLds = 32;    // traversal stack entries per lane in LDS (megakernel)
-constexpr int kStackLdsQueue = 16;  // LDS entries of the queue-based traversal kernels
+#ifndef PRT_QUEUE_STACK_LDS
+#define PRT_QUEUE_STACK_LDS 16
+#endif
+constexpr int kStackLdsQueue = PRT_QUEUE_STACK_LDS;  // LDS entries of the queue-based traversal kernels
 constexpr int kStackMax = 128;   // total entries (the rest in the global spill area)
 
 // Device view of a built scene. Layouts:

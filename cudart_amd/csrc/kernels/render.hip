@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock, 4) void megakernel(MegaArgs a) {
     Counts cnt = {0, 0, 0, 0, 0, 0};
     if (x < int(a.width) && y < int(a.height)) {
         Stack st;
-        st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
+        st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
         const size_t idx = size_t(y) * a.width + size_t(x);
         float4 acc = a.accum[idx];
         V3 sum = v3(acc.x, acc.y, acc.z);
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void first_hit_kernel(FirstHitArgs a) {
     const int y = int(tile / a.tiles_x) * 8 + (lane >> 3);
     if (x >= int(a.width) || y >= int(a.height)) return;
     Stack st;
-    st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
+    st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
     Rng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), a.sample};
     V3 ro, rd;
     camera_ray(a.cam, x, y, rng.block(0), ro, rd);

@@ -199,7 +199,11 @@ struct LaneFeeder {
 // Idle lanes of a wave that trigger a refill (the rest keep traversing meanwhile).
 // kRefill <= 64, so a wave with no active lane always refills or, with no work
 // left, leaves the loop.
-constexpr uint32_t kRefill = 16;
+#ifndef PRT_REFILL_LANES
+#define PRT_REFILL_LANES 16
+#endif
+constexpr uint32_t kRefill = PRT_REFILL_LANES;
+static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold out of range");
 
 __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__popcll(__ballot(!active))); }
 
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
     }
     const uint32_t lane = threadIdx.x;
     StackT<kStackLdsQueue> st;
-    st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
+    st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
     LaneFeeder feed;
     feed.start(&a.ctr[kFetch + (3 * it) * kSpread], n);
@@ -332,7 +336,8 @@ __global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
         }
         // One node or leaf per active lane; a refill is considered every step.
         if (active) {
-            if (trav_step<false, COUNT>(nodes, tris, ray, kEps, h, st, cnt) != kTravRunning) {
+            if (trav_step<false, COUNT, true>(nodes, tris, ray, kEps, h, st, cnt) != kTravRunning) {
+                if (h.rec != 0xFFFFFFFFu) hit_details(tris, ray, h);
                 a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.rec));
                 a.hmat[i] = h.mat;
                 active = false;
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     const uint32_t n = a.ctr[kShadow];
     const uint32_t lane = threadIdx.x;
     StackT<kStackLdsQueue> st;
-    st.bind(s_stack + lane, a.spill + size_t(blockIdx.x) * kBlock + lane, a.spill_stride);
+    st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
     LaneFeeder feed;
     feed.start(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
@@ -503,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
             }
         }
         if (active) {
-            const int r = trav_step<true, COUNT>(nodes, tris, ray, kEps, occ, st, cnt);
+            const int r = trav_step<true, COUNT, true>(nodes, tris, ray, kEps, occ, st, cnt);
             if (r != kTravRunning) {
                 const float4 c = *dst;
                 const V3 s = add_direct(v3(c), mask, direct, r == kTravDone);

@@ -1,5 +1,6 @@
-"""BVH invariants (product four-wide layout and the oracle's reference-shaped
-tree) and closest-hit equality with an O(N) brute-force scan."""
+"""BVH invariants (product four-wide layout with quantized child boxes, and the
+oracle's reference-shaped tree) and closest-hit equality with an O(N) brute-force
+scan."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,8 @@ def _check_product_layout(mesh):
     np.testing.assert_array_equal(recs[:, 8:11], p[:, 2] - p[:, 0])
     assert np.array_equal(recs[:, 3].view(np.uint32), order)
     assert np.array_equal(recs[:, 7].view(np.uint32), a.material_ids[order])
-    words = nodes[:, 24:28].view(np.int32)
+    u32 = nodes.view(np.uint32)
+    words = nodes[:, 4:8].view(np.int32)
     lo, hi = _tri_bounds(a.positions)
     covered = np.zeros(n, np.int32)
     seen_nodes = np.zeros(len(nodes), np.int32)
@@ -36,20 +38,24 @@ def _check_product_layout(mesh):
         k, plo, phi, depth = stack.pop()
         seen_nodes[k] += 1
         depth_max = max(depth_max, depth)
-        nd = nodes[k]
-        assert (nd[28:32] == 0).all()
+        origin = nodes[k, 0:3].astype(np.float32)
+        ex = int(u32[k, 3])
+        step = np.array([np.uint32(((ex >> (8 * ax)) & 0xFF) << 23).view(np.float32) for ax in range(3)], np.float32)
+        assert (u32[k, 14:16] == 0).all() and (ex >> 24) == 0
         used = 0
         for c in range(4):
-            clo = np.array([nd[c], nd[8 + c], nd[16 + c]])
-            chi = np.array([nd[4 + c], nd[12 + c], nd[20 + c]])
             w = int(words[k, c])
-            if w == 0x7FFFFFFF:  # empty slot: zero box, only after the used ones
-                assert (clo == 0).all() and (chi == 0).all()
+            q = np.array([(int(u32[k, 8 + j]) >> (8 * c)) & 0xFF for j in range(6)], np.float32)
+            if w == 0x7FFFFFFF:  # empty slot: no planes, only after the used ones
+                assert (q == 0).all()
                 continue
             assert used == c, "empty slots come last"
             used += 1
+            # decode as the kernel does: fmaf(q, 2^e, origin); q * 2^e is exact, so
+            # the float32 sum is the single rounding of the fused operation
+            clo = (q[0::2] * step + origin).astype(np.float32)
+            chi = (q[1::2] * step + origin).astype(np.float32)
             assert (clo <= chi).all()
-            assert (clo >= plo).all() and (chi <= phi).all(), "child box outside its parent's"
             if w >= 0:
                 assert w < len(nodes)
                 stack.append((w, clo, chi, depth + 1))
@@ -62,6 +68,8 @@ def _check_product_layout(mesh):
                 assert (lo[ids] > clo).all() and (hi[ids] < chi).all(), "leaf box is not padded"
                 covered[first:first + count] += 1
         assert used >= 1 and (used >= 2 or len(nodes) == 1), "an inner node holds at least two children"
+        # the children's decoded boxes lie inside the parent's decoded box for this node
+        assert (origin >= plo).all()
     assert (covered == 1).all(), "every record in exactly one leaf"
     assert (seen_nodes == 1).all(), "every node reached once"
     assert depth_max == info.max_depth
