@@ -40,6 +40,8 @@ constexpr int kMaxIter = 64;
 // aggregated per workgroup.
 constexpr uint32_t kSpread = 64;
 constexpr int kShadeBlock = 512;
+// Bytes of queues and path state per slot: 2 queues x 64, hit 16 + 4, shadow 64, result 16.
+constexpr size_t kSlotBytes = 2 * 64 + 16 + 4 + 64 + 16;
 enum WfCounter : uint32_t {
     kExt0 = 0,      // ray count of queue 0
     kExt1 = 1,      // ray count of queue 1
@@ -625,8 +627,7 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     w.spill = nullptr;
     w.ctr = nullptr;
     // per slot: 2 queues x 64 B, hit 16 + 4, shadow 64, result 16
-    const size_t per = 2 * 64 + 16 + 4 + 64 + 16;
-    w.bytes = per * size_t(capacity);
+    w.bytes = kSlotBytes * size_t(capacity);
     WF_TRY(hipMalloc(&w.mem, w.bytes));
     WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.ctr), kNumCounters * sizeof(uint32_t)));
     WF_TRY(hipMemset(w.ctr, 0, kNumCounters * sizeof(uint32_t)));
@@ -837,9 +838,17 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     return PRT_OK;
 }
 
+// The most path slots worth holding: a quarter of the device's free memory (an
+// MI355X's 288 GB holds a whole 1080p x 128-spp frame, 265M paths, in one chunk),
+// at most 2^28. Fewer, larger chunks leave fewer nearly empty late iterations.
+uint64_t max_slots() {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = size_t(8) << 30;
+    return std::max<uint64_t>(1u << 20, std::min<uint64_t>(1ull << 28, free_b / 4 / kSlotBytes));
+}
+
 uint32_t default_capacity(uint64_t total_paths) {
-    const uint64_t cap = 1ull << 23;  // 8M slots (about 1.6 GB of queues and state)
-    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(cap, total_paths)));
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(max_slots(), total_paths)));
 }
 
 }  // namespace
@@ -858,8 +867,8 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     const uint64_t total = uint64_t(r->w) * r->h * n;
     const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
     RunStats rs;
-    int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap, 0,
-                           r->cfg.count_rays != 0, &rs, r->cfg.profile_kernels != 0);
+    int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap,
+                           r->cfg.wavefront_blocks_per_cu, r->cfg.count_rays != 0, &rs, r->cfg.profile_kernels != 0);
     if (rc != PRT_OK) return rc;
     // Fold the wavefront's counters into the renderer's.
     unsigned long long host[kCntNum];
@@ -1014,22 +1023,33 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
         batch_release(b);
         return rc;
     }
+    uint64_t full = 0;
+    for (uint32_t i = 0; i < n; ++i) full += uint64_t(jobs[i].camera.xpixels) * jobs[i].camera.ypixels * jobs[i].spp;
     hipStream_t s;
     WF_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     WavefrontState w;
     double bms = 1e30;
     prt_tune bt{};
-    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
-    const uint32_t bpcs[] = {4, 8, 12, 16};
-    for (uint32_t cap : caps) {
-        if (cap > 2 * paths && cap != caps[0]) continue;
+    // Candidates: the number of chunks the full workload is cut into (the trial is
+    // cut into as many), and resident traversal workgroups per compute unit.
+    const uint64_t limit = max_slots();
+    const uint32_t splits[] = {1, 2, 4, 8, 16};
+    const uint32_t bpcs[] = {12, 16, 20, 24};
+    uint32_t tried_min = 0;
+    for (uint32_t k : splits) {
+        const uint64_t want = std::max<uint64_t>((full + k - 1) / k, 1);
+        if (want > limit && k != splits[4]) continue;  // does not fit: more chunks
+        const uint32_t trial_cap = uint32_t(std::max<uint64_t>(1, (paths + k - 1) / k));
+        const uint32_t full_cap = uint32_t(std::min<uint64_t>(want, limit));
+        if (tried_min && full_cap >= tried_min) continue;
+        tried_min = full_cap;
         for (uint32_t bpc : bpcs) {
             RunStats rs;
-            rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, cap, bpc, false, &rs);
+            rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, trial_cap, bpc, false, &rs);
             if (rc != PRT_OK) break;
             if (rs.ms < bms) {
                 bms = rs.ms;
-                bt = prt_tune{cap, 0, kBlock, bpc};
+                bt = prt_tune{full_cap, 0, kBlock, bpc};
             }
         }
         if (rc != PRT_OK) break;

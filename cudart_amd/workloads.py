@@ -88,3 +88,19 @@ def batch_plan(seed: int = SEED, views: int = 64, scenes: int = 8):
         spp = int(rng.choice([8, 32, 128, 256]))
         plan.append((s, side, spp))
     return [int(t) for t in tris], plan
+
+
+def batch(seed: int = SEED, views: int = 64, scenes: int = 8, max_depth: int = 8):
+    """The batched workload: `scenes` sphere scenes of batch_plan's sizes and `views`
+    square views of them from cameras on a circle around the scene centre, each with
+    its own resolution, spp and view id. Returns (meshes, [(scene index, Camera, spp,
+    view_id)], max_depth)."""
+    tris, plan = batch_plan(seed, views, scenes)
+    meshes = [Mesh.generate("mixed", f"tris={t}", seed + 101 * i) for i, t in enumerate(tris)]
+    jobs = []
+    for v, (s, side, spp) in enumerate(plan):
+        th = 2.0 * np.pi * v / views
+        eye = (35.0 * np.sin(th), 12.0 + 4.0 * np.cos(3.0 * th), 35.0 * np.cos(th))
+        cam = camera_look_at(eye, (0.0, 8.0, 0.0), (0.0, 1.0, 0.0), 45.0, side, side)
+        jobs.append((s, cam, spp, v + 1))
+    return meshes, jobs, max_depth

@@ -121,6 +121,8 @@ typedef struct prt_render_config {
     uint32_t wavefront_queue;   /* wavefront: path slots in flight (0 = choose) */
     uint32_t profile_kernels;   /* wavefront: 1 = launch each chunk's kernels without a graph and
                                    time every extend / shade / shadow launch with events */
+    uint32_t wavefront_blocks_per_cu; /* wavefront: resident traversal workgroups per compute unit
+                                         (0 = as many as fit) */
 } prt_render_config;
 
 /* Counters accumulated since the renderer was created (or reset). */
