@@ -42,8 +42,11 @@ struct BvhLayout {
     float sah_cost = 0.f;
 };
 
+#ifndef PRT_BVH_LEAF_SIZE
+#define PRT_BVH_LEAF_SIZE 4
+#endif
 struct BvhParams {
-    int leaf_size = 4;        // SAH leaves hold at most this many triangles
+    int leaf_size = PRT_BVH_LEAF_SIZE;  // SAH leaves hold at most this many triangles
     int sah_depth_cap = 64;   // below this depth, object-median splits only
     float pad_rel = 1.0f / 128.0f;  // leaf box padding relative to its extent
 };

@@ -28,6 +28,10 @@ def test_full_batch_equals_per_view_megakernel(gpu):
     tune_s = time.perf_counter() - t0
     imgs, st = ca.render_batch(jobs, tune=tune, max_depth=depth)
     batch_ms = st.last_render_ms
+    imgs0, st0 = ca.render_batch(jobs, max_depth=depth)  # default queue and occupancy
+    for a, b in zip(imgs, imgs0):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    print(f"\n[batch] default tuning: {st0.last_render_ms:.0f} ms", flush=True)
     print(f"\n[batch] {len(jobs)} views, {sum(m.num_triangles for m in meshes)} triangles in {len(scenes)} scenes, "
           f"{paths / 1e9:.2f} G paths; autotune {tune_s:.1f} s -> queue {tune.queue_slots}, "
           f"{tune.blocks_per_cu} blocks/CU; batch {batch_ms:.0f} ms ({paths / batch_ms / 1e6:.2f} G paths/s)",
