@@ -28,7 +28,8 @@ the committed summary under profiles/, null when absent).
 
 Adjoint leg: path-replay gradients of the L2 image loss w.r.t. 16 materials'
 albedo and emission on the 100k-triangle Cornell-like scene at 512x512, 64
-samples replayed: grad-samples/s = pixels x samples / adjoint device time.
+samples replayed (the forward render keeps each sample's radiance): grad-samples/s
+= pixels x samples / adjoint device time.
 
 CPU baseline: the oracle (C++ restatement of the reference --cpu renderer, fp32,
 OpenMP) on a bounded sample of pixels of the same frame, in Mrays/s with the
@@ -163,7 +164,8 @@ def adjoint_leg():
     rt.close()
     for i in range(M):
         scene.set_material_params(i, kd0[i], float(em0[i]))
-    r = ca.ParallelRenderer(scene, 512, 512, 64, seed=W.SEED, **kw)
+    # The forward pass keeps each sample's radiance, so the adjoint only replays paths.
+    r = ca.ParallelRenderer(scene, 512, 512, 64, seed=W.SEED, keep_samples=1, **kw)
     r.render(64)
     image = r.radiance()
     r.adjoint(target, 64, image=image)  # warm-up

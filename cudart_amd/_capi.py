@@ -50,7 +50,8 @@ class RenderConfig(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32), ("view_id", C.c_uint32),
                 ("device", C.c_int32), ("integrator", C.c_uint32), ("spp_per_launch", C.c_uint32),
                 ("count_rays", C.c_uint32), ("wavefront_chunk", C.c_uint32), ("wavefront_queue", C.c_uint32),
-                ("profile_kernels", C.c_uint32), ("wavefront_blocks_per_cu", C.c_uint32)]
+                ("profile_kernels", C.c_uint32), ("wavefront_blocks_per_cu", C.c_uint32),
+                ("keep_samples", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -4,10 +4,10 @@
 //
 // Given dLoss/dI for every pixel (I = mean of `spp` samples), each sample's
 // radiance L has adjoint delta = dLoss/dI / spp. For each pixel-sample the kernel
-// traces the path once (primal, the
-// megakernel's arithmetic) to get its radiance L, then replays it with the same
-// Philox slots, keeping the radiance still to be collected, rem = L - (what the
-// path collected so far). Parameters:
+// takes L from the forward pass when the renderer kept it (keep_samples), or traces
+// the path once (primal, the megakernel's arithmetic) to get it, then replays the
+// path with the same Philox slots, keeping the radiance still to be collected,
+// rem = L - (what the path collected so far). Parameters:
 //   kd_m (RGB)  a diffuse vertex multiplies everything collected after it by
 //               kd_c, so it contributes delta_c * rem_c / kd_c (rem taken after
 //               that vertex's emission, before its next-event estimate);
@@ -42,6 +42,7 @@ struct AdjArgs {
     uint32_t width, height, spp;
     uint32_t tiles_x, tiles;
     const float* grad;    // W*H*3 dLoss / dI
+    const float4* samples;  // nullptr, or each sample's radiance from the forward pass [s * W * H + p]
     double norm;          // 1 / spp: dI / dL of one sample
     double half_hess;     // (d2 Loss / dI2) / (2 spp^2): loss curvature per sample radiance
     uint32_t num_params;  // 4 * materials: kd (M*3) then emission (M)
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
             const D3 delta{a.norm * double(a.grad[idx]), a.norm * double(a.grad[idx + 1]),
                            a.norm * double(a.grad[idx + 2])};
             for (uint32_t s = 0; s < a.spp; ++s) {
-                const V3 L = primal(a, x, y, s, st, cnt);
+                const V3 L = a.samples ? v3(a.samples[size_t(s) * a.width * a.height + idx / 3])
+                                       : primal(a, x, y, s, st, cnt);
                 replay(a, x, y, s, delta, L, s_g, st, cnt);
             }
         }
@@ -302,6 +304,13 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     a.tiles_x = r->tiles_x;
     a.tiles = r->tiles;
     a.grad = d_grad;
+    // The forward pass's per-sample radiance, when it was kept for exactly these
+    // samples with the scene's current materials (same seed, view and depth: the
+    // renderer's configuration).
+    a.samples = r->samples && r->samples_ok && r->samples_done >= spp && spp <= r->spp &&
+                        r->samples_version == r->scene->mats_version
+                    ? r->samples
+                    : nullptr;
     a.norm = 1.0 / double(spp);
     a.half_hess = double(hess) / (2.0 * double(spp) * double(spp));
     a.num_params = P;

@@ -210,6 +210,8 @@ extern "C" int prt_renderer_create(prt_scene* scene, const prt_camera* cam, uint
     gamma_table(lut);
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&r->gamma_lut), 256));
     HIP_TRY(hipMemcpy(r->gamma_lut, lut, 256, hipMemcpyHostToDevice));
+    if (c.keep_samples && spp > 0)
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&r->samples), px * spp * sizeof(float4)));
     r->tiles_x = (w + 7) / 8;
     r->tiles = r->tiles_x * ((h + 7) / 8);
     rc = ensure_spill(r.get(), ((r->tiles + 7) / 8) * 8 * kBlock);
@@ -228,6 +230,14 @@ extern "C" int prt_render_spp(prt_renderer* r, uint32_t n) {
     if (rc != PRT_OK) return rc;
     rc = device_scene(r->scene, r->cfg.device, r->sc);
     if (rc != PRT_OK) return rc;
+    if (r->samples) {
+        if (r->samples_done == 0) {
+            r->samples_ok = true;
+            r->samples_version = r->scene->mats_version;
+        } else if (r->samples_version != r->scene->mats_version) {
+            r->samples_ok = false;  // this accumulation mixes material settings
+        }
+    }
     if (r->cfg.integrator == PRT_INTEGRATOR_WAVEFRONT) return render_wavefront(r, n);
     MegaArgs a;
     a.sc = r->sc;
@@ -238,6 +248,8 @@ extern "C" int prt_render_spp(prt_renderer* r, uint32_t n) {
     a.tiles_x = r->tiles_x;
     a.tiles = r->tiles;
     a.accum = r->accum;
+    a.samples = r->samples;
+    a.samples_cap = r->samples ? r->spp : 0;
     a.counters = r->counters;
     a.spill = r->spill;
     a.spill_stride = r->spill_threads;
@@ -275,6 +287,7 @@ extern "C" int prt_renderer_reset(prt_renderer* r) {
     HIP_TRY(hipMemsetAsync(r->accum, 0, size_t(r->w) * r->h * sizeof(float4), r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
     r->samples_done = 0;
+    r->samples_ok = false;
     return PRT_OK;
 }
 
@@ -398,7 +411,7 @@ extern "C" int prt_renderer_destroy(prt_renderer* r) {
     (void)hipSetDevice(r->cfg.device);
     release_wavefront(r);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
-    void* ptrs[] = {r->accum, r->counters, r->gamma_lut, r->spill};
+    void* ptrs[] = {r->accum, r->counters, r->gamma_lut, r->spill, r->samples};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (r->ev0) (void)hipEventDestroy(r->ev0);

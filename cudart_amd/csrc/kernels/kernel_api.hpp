@@ -69,6 +69,8 @@ struct MegaArgs {
     uint32_t s0, nspp;
     uint32_t tiles_x, tiles;  // 8x8 tiles
     float4* accum;            // width*height, .xyz = running sum
+    float4* samples;          // nullptr, or each sample's radiance at [sample * width * height + pixel]
+    uint32_t samples_cap;     // samples stored per pixel
     unsigned long long* counters;  // kCntNum or nullptr
     int* spill;               // (kStackMax - kStackLds) * spill_stride ints
     uint32_t spill_stride;

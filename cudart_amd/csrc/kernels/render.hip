@@ -87,6 +87,8 @@ __global__ __launch_bounds__(kBlock, 4) void megakernel(MegaArgs a) {
         for (uint32_t k = 0; k < a.nspp; ++k) {
             V3 L = sample_path<COUNT>(a.sc, a.cam, a.path, x, y, a.s0 + k, st, cnt);
             sum = sum + L;
+            if (a.samples && a.s0 + k < a.samples_cap)
+                a.samples[size_t(a.s0 + k) * a.width * a.height + idx] = make_float4(L.x, L.y, L.z, 0.f);
         }
         a.accum[idx] = make_float4(sum.x, sum.y, sum.z, 0.f);
     }

@@ -32,6 +32,12 @@ struct prt_renderer {
     double kernel_ms[3] = {0, 0, 0};  // profile_kernels: extend, shade, shadow of the last wavefront render
     uint32_t kernel_launches[3] = {0, 0, 0};
     prt::WavefrontState* wf = nullptr;
+    // keep_samples: radiance of pixel-sample (s, p) at samples[s * w * h + p], s < spp.
+    // Valid for the adjoint while every sample since the last reset was rendered with
+    // the scene's current materials (samples_version).
+    float4* samples = nullptr;
+    bool samples_ok = false;
+    uint64_t samples_version = 0;
 };
 
 namespace prt {

@@ -59,6 +59,8 @@ struct WfJob {
     PathArgs path;
     uint32_t width, height;
     float4* accum;
+    float4* samples;       // nullptr, or each sample's radiance at [sample * width * height + pixel]
+    uint32_t samples_cap;  // samples stored per pixel
 };
 
 struct WfTile {
@@ -531,9 +533,12 @@ __global__ __launch_bounds__(256) void wf_resolve(WfArgs a) {
         float4* acc = a.jobs[t.job].accum + p;
         float4 s = *acc;
         V3 sum = v3(s.x, s.y, s.z);
+        const WfJob& jb = a.jobs[t.job];
+        const size_t npx = size_t(jb.width) * jb.height;
         for (uint32_t smp = t.s0; smp < t.s1; ++smp) {
             const float4 r = a.result[t.slot0 + (smp - t.s0) * w + (p - t.p0)];
             sum = sum + v3(r);
+            if (jb.samples && smp < jb.samples_cap) jb.samples[size_t(smp) * npx + p] = make_float4(r.x, r.y, r.z, 0.f);
         }
         *acc = make_float4(sum.x, sum.y, sum.z, 0.f);
     }
@@ -862,6 +867,8 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     j.width = r->w;
     j.height = r->h;
     j.accum = r->accum;
+    j.samples = r->samples;
+    j.samples_cap = r->samples ? r->spp : 0;
     std::vector<WfJob> jobs = {j};
     std::vector<PlanJob> pj = {PlanJob{r->w * r->h, r->samples_done, r->samples_done + n}};
     const uint64_t total = uint64_t(r->w) * r->h * n;
@@ -923,6 +930,8 @@ int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCt
         b.accum.push_back(acc);
         WF_TRY(hipMemset(acc, 0, px * sizeof(float4)));
         w.accum = acc;
+        w.samples = nullptr;
+        w.samples_cap = 0;
         b.jobs.push_back(w);
         b.pjobs.push_back(PlanJob{uint32_t(px), 0, jb.spp});
         b.total += uint64_t(px) * jb.spp;

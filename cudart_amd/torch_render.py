@@ -6,7 +6,8 @@ renderer's seed); backward runs the path-replay adjoint kernel with the incoming
 dLoss/dImage. Russian-roulette decisions are differentiated through their survival
 probability; the loss change of such a decision is taken to second order with the
 curvature `hess` (d2Loss/dI2 per element: 2/(H*W*3) for the mean squared error, 0 for
-a first-order estimate).
+a first-order estimate). A renderer created with keep_samples=1 stores each sample's
+radiance in the forward pass, and backward then only replays the paths.
 """
 from __future__ import annotations
 

@@ -123,6 +123,9 @@ typedef struct prt_render_config {
                                    time every extend / shade / shadow launch with events */
     uint32_t wavefront_blocks_per_cu; /* wavefront: resident traversal workgroups per compute unit
                                          (0 = as many as fit) */
+    uint32_t keep_samples;  /* 1 = also store each pixel-sample's radiance (16 B per pixel-sample, for
+                               the first `spp` samples), so the adjoint replays paths without
+                               tracing them first */
 } prt_render_config;
 
 /* Counters accumulated since the renderer was created (or reset). */
