@@ -217,9 +217,7 @@ __device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
     wa = w;
 }
 __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xFFu); }
-// ORDERED = false (occlusion rays, where any order finds an occluder) skips the
-// sort and visits the hit children in slot order.
-template <bool ORDERED, class StackType>
+template <class StackType>
 __device__ __forceinline__ int visit_node(const float4* nodes, int node, float ix, float iy, float iz, float ox,
                                           float oy, float oz, float tmin, float tfar, StackType& st) {
     const size_t nb = size_t(node) * 4;
@@ -248,16 +246,7 @@ __device__ __forceinline__ int visit_node(const float4* nodes, int node, float i
         // decoded planes live (register pressure, hence occupancy).
         __builtin_amdgcn_sched_barrier(0);
     }
-    if (!ORDERED) {
-        int first = kNoChild;
-#pragma unroll
-        for (int c = 3; c >= 0; --c) {
-            if (word[c] == kNoChild) continue;
-            if (first != kNoChild) st.push(first);
-            first = word[c];
-        }
-        return first;
-    }
+    // Occlusion rays use the same order: the nearest boxes hold occluders most often.
     cswap(key[0], word[0], key[1], word[1]);
     cswap(key[2], word[2], key[3], word[3]);
     cswap(key[0], word[0], key[2], word[2]);
@@ -286,7 +275,7 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
     while (true) {
         if (node >= 0) {
             if (COUNT) cnt.nodes++;
-            node = visit_node<!ANY>(sc.nodes, node, ix, iy, iz, ox, oy, oz, tmin, best.t, st);
+            node = visit_node(sc.nodes, node, ix, iy, iz, ox, oy, oz, tmin, best.t, st);
             if (node != kNoChild) continue;
         } else {
             const uint32_t w = uint32_t(node);
@@ -352,8 +341,7 @@ __device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris
     const int node = r.node;
     if (node >= 0) {
         if (COUNT) cnt.nodes++;
-        r.node = visit_node<!ANY>(nodes, node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz, tmin, best.t,
-                                  st);
+        r.node = visit_node(nodes, node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz, tmin, best.t, st);
         if (r.node != kNoChild) return kTravRunning;
     } else {
         const uint32_t w = uint32_t(node);

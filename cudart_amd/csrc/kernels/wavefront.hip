@@ -1042,8 +1042,10 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     // Candidates: the number of chunks the full workload is cut into (the trial is
     // cut into as many), and resident traversal workgroups per compute unit.
     const uint64_t limit = max_slots();
+    // (A short trial favours fewer resident workgroups than a full run does, so the
+    // candidates stay near full occupancy: 24 is what fits of the traversal kernels.)
     const uint32_t splits[] = {1, 2, 4, 8, 16};
-    const uint32_t bpcs[] = {12, 16, 20, 24};
+    const uint32_t bpcs[] = {16, 20, 24};
     uint32_t tried_min = 0;
     for (uint32_t k : splits) {
         const uint64_t want = std::max<uint64_t>((full + k - 1) / k, 1);
