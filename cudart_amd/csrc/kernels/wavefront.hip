@@ -1018,11 +1018,11 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     int rc = set_device(c.device);
     if (rc != PRT_OK) return rc;
     PRT_TRY_BEGIN
-    // Trial workload: every job at min(spp, 2) samples per pixel.
+    // Trial workload: every job at min(spp, 4) samples per pixel.
     std::vector<prt_job> trial(jobs, jobs + n);
     uint64_t paths = 0;
     for (auto& t : trial) {
-        t.spp = std::min<uint32_t>(t.spp, 2);
+        t.spp = std::min<uint32_t>(t.spp, 4);
         t.out_rgb = nullptr;
         paths += uint64_t(t.camera.xpixels) * t.camera.ypixels * t.spp;
     }
@@ -1039,31 +1039,27 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     WavefrontState w;
     double bms = 1e30;
     prt_tune bt{};
-    // Candidates: the number of chunks the full workload is cut into (the trial is
-    // cut into as many), and resident traversal workgroups per compute unit.
+    // Queue depth: the fewest chunks of the full work that fit the path slots (larger
+    // chunks were faster at every size measured, since every chunk ends in nearly empty
+    // iterations). Occupancy: timed on the trial, cut into chunks of that same size
+    // (or one chunk when the trial is smaller), so each trial chunk behaves like a full
+    // one. The candidates stay near full occupancy: 24 workgroups per CU is what fits.
     const uint64_t limit = max_slots();
-    // (A short trial favours fewer resident workgroups than a full run does, so the
-    // candidates stay near full occupancy: 24 is what fits of the traversal kernels.)
-    const uint32_t splits[] = {1, 2, 4, 8, 16};
+    const uint32_t full_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full, limit)));
+    const uint32_t trial_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full_cap, paths)));
     const uint32_t bpcs[] = {16, 20, 24};
-    uint32_t tried_min = 0;
-    for (uint32_t k : splits) {
-        const uint64_t want = std::max<uint64_t>((full + k - 1) / k, 1);
-        if (want > limit && k != splits[4]) continue;  // does not fit: more chunks
-        const uint32_t trial_cap = uint32_t(std::max<uint64_t>(1, (paths + k - 1) / k));
-        const uint32_t full_cap = uint32_t(std::min<uint64_t>(want, limit));
-        if (tried_min && full_cap >= tried_min) continue;
-        tried_min = full_cap;
-        for (uint32_t bpc : bpcs) {
+    for (uint32_t bpc : bpcs) {
+        double t = 1e30;
+        for (int rep = 0; rep < 2 && rc == PRT_OK; ++rep) {  // best of two
             RunStats rs;
             rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, trial_cap, bpc, false, &rs);
-            if (rc != PRT_OK) break;
-            if (rs.ms < bms) {
-                bms = rs.ms;
-                bt = prt_tune{full_cap, 0, kBlock, bpc};
-            }
+            t = std::min(t, rs.ms);
         }
         if (rc != PRT_OK) break;
+        if (t < bms) {
+            bms = t;
+            bt = prt_tune{full_cap, 0, kBlock, bpc};
+        }
     }
     free_state(w);
     (void)hipStreamDestroy(s);
