@@ -311,14 +311,27 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
 }
 
 // Resumable form of traverse() for kernels that refill finished lanes with new
-// rays: the same visiting order, culling and (t, id) ordering, one node or leaf
-// per call. trav_begin() sets up a ray; trav_step() advances it and returns
-// kTravRunning, kTravDone (closest hit: best holds the result; any-hit: no
-// occluder) or kTravOccluded (any-hit only).
+// rays: the same culling and (t, id) ordering, one wave-wide phase per call.
+//
+// Leaves are postponed (Aila & Laine, "Understanding the efficiency of ray
+// traversal on GPUs", HPG 2009): a lane that reaches a leaf sets it aside and
+// goes on visiting inner nodes; it stops only at a second leaf. The caller picks
+// the phase for the whole wave: a node phase while some lane has no leaf set
+// aside, else a leaf phase, in which every lane tests its leaf. Node decoding and
+// triangle tests then never share a step, so a step waits on one kind of load.
+// A set-aside leaf is tested after nodes that a tested leaf might have culled;
+// that costs visits, never results: the (t, id) minimum does not depend on order.
+// Passing node_phase = trav_wants_nodes(r) per lane gives the unphased order: a
+// lane tests each leaf at the step after it reaches it.
+//
+// trav_begin() sets up a ray; trav_step() returns kTravRunning, kTravDone
+// (closest hit: best holds the result; any-hit: no occluder) or kTravOccluded
+// (any-hit only).
 struct TravRay {
     V3 o, d;
     float ix, iy, iz;  // o * i is recomputed per node (the same fp32 products as traverse's)
-    int node;
+    int node;          // the inner node to visit next, a second leaf, or kNoChild
+    int leaf;          // the leaf set aside for the next leaf phase, or kNoChild
 };
 enum { kTravRunning = 0, kTravDone = 1, kTravOccluded = 2 };
 
@@ -330,23 +343,31 @@ __device__ __forceinline__ bool trav_begin(TravRay& r, V3 o, V3 d) {
     r.iy = 1.f / d.y;
     r.iz = 1.f / d.z;
     r.node = 0;
+    r.leaf = kNoChild;
     return !(d.x == 0.f && d.y == 0.f && d.z == 0.f);
 }
+
+// True for a lane that a node phase can advance without a leaf set aside.
+__device__ __forceinline__ bool trav_wants_nodes(const TravRay& r) { return r.leaf == kNoChild; }
 
 // LEAN keeps only t, rec and id of the best hit (u, v and the material are not
 // written; hit_details() recomputes them from the record afterwards).
 template <bool ANY, bool COUNT, bool LEAN, class StackType>
-__device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris, TravRay& r, float tmin, HitRec& best,
-                                         StackType& st, Counts& cnt) {
-    const int node = r.node;
-    if (node >= 0) {
-        if (COUNT) cnt.nodes++;
-        r.node = visit_node(nodes, node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz, tmin, best.t, st);
-        if (r.node != kNoChild) return kTravRunning;
-    } else {
-        const uint32_t w = uint32_t(node);
+__device__ __forceinline__ int trav_step(bool node_phase, const float4* nodes, const float4* tris, TravRay& r,
+                                         float tmin, HitRec& best, StackType& st, Counts& cnt) {
+    if (node_phase) {
+        if (uint32_t(r.node) < uint32_t(kNoChild)) {  // an inner node
+            if (COUNT) cnt.nodes++;
+            int next = visit_node(nodes, r.node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz, tmin,
+                                  best.t, st);
+            if (next == kNoChild && st.sp != 0) next = st.pop();
+            r.node = next;
+        }
+    } else if (r.leaf != kNoChild) {
+        const uint32_t w = uint32_t(r.leaf);
         const uint32_t first = w & 0x0FFFFFFFu;
         const uint32_t count = ((w >> 28) & 7u) + 1u;
+        r.leaf = kNoChild;
         for (uint32_t k = 0; k < count; ++k) {
             const size_t tb = size_t(first + k) * 3;
             float4 a = gload(tris, tb), b = gload(tris, tb + 1), c = gload(tris, tb + 2);
@@ -358,6 +379,7 @@ __device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris
             if (t < best.t || (t == best.t && id < best.id)) {
                 if (ANY) {
                     st.sp = 0;
+                    r.node = kNoChild;
                     return kTravOccluded;
                 }
                 best.t = t;
@@ -371,9 +393,12 @@ __device__ __forceinline__ int trav_step(const float4* nodes, const float4* tris
             }
         }
     }
-    if (st.sp == 0) return kTravDone;
-    r.node = st.pop();
-    return kTravRunning;
+    // Set a reached leaf aside (at most one) and continue with the next entry.
+    if (r.leaf == kNoChild && r.node < 0) {
+        r.leaf = r.node;
+        r.node = st.sp != 0 ? st.pop() : kNoChild;
+    }
+    return (r.leaf == kNoChild && r.node == kNoChild) ? kTravDone : kTravRunning;
 }
 
 // u, v and material of best.rec, by the intersection test that selected it
