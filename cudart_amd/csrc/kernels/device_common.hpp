@@ -258,58 +258,6 @@ __device__ __forceinline__ int visit_node(const float4* nodes, int node, float i
     return word[0];
 }
 
-// Closest hit (ANY = false) or occlusion test against a known best (ANY = true).
-// Accepts t in (tmin, best.t) or t == best.t with a lower original id: the
-// closest hit is the lexicographic minimum of (t, original id), independent of
-// tree shape and visiting order. Returns true when best changed (closest) or
-// when an occluder was found (any).
-template <bool ANY, bool COUNT, class StackType>
-__device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float tmin, HitRec& best, StackType& st,
-                                         Counts& cnt) {
-    bool found = false;
-    if (d.x == 0.f && d.y == 0.f && d.z == 0.f) return false;  // det == 0 for every triangle
-    const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-    const int base = st.sp;
-    int node = 0;
-    while (true) {
-        if (node >= 0) {
-            if (COUNT) cnt.nodes++;
-            node = visit_node(sc.nodes, node, ix, iy, iz, ox, oy, oz, tmin, best.t, st);
-            if (node != kNoChild) continue;
-        } else {
-            const uint32_t w = uint32_t(node);
-            const uint32_t first = w & 0x0FFFFFFFu;
-            const uint32_t count = ((w >> 28) & 7u) + 1u;
-            for (uint32_t k = 0; k < count; ++k) {
-                const size_t tb = size_t(first + k) * 3;
-                float4 a = gload(sc.tris, tb), b = gload(sc.tris, tb + 1), c = gload(sc.tris, tb + 2);
-                if (COUNT) cnt.tris++;
-                float t, u, v;
-                if (!mt_intersect(a, b, c, o, d, t, u, v)) continue;
-                uint32_t id = __float_as_uint(a.w);
-                if (!(t > tmin)) continue;
-                if (t < best.t || (t == best.t && id < best.id)) {
-                    if (ANY) {
-                        st.sp = base;
-                        return true;
-                    }
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                    best.rec = first + k;
-                    best.id = id;
-                    best.mat = __float_as_uint(b.w);
-                    found = true;
-                }
-            }
-        }
-        if (st.sp == base) break;
-        node = st.pop();
-    }
-    return found;
-}
-
 // Resumable form of traverse() for kernels that refill finished lanes with new
 // rays: the same culling and (t, id) ordering, one wave-wide phase per call.
 //
@@ -399,6 +347,50 @@ __device__ __forceinline__ int trav_step(bool node_phase, const float4* nodes, c
         r.node = st.sp != 0 ? st.pop() : kNoChild;
     }
     return (r.leaf == kNoChild && r.node == kNoChild) ? kTravDone : kTravRunning;
+}
+
+// Phase choice for a traversal step, over the lanes of the wave that make the call.
+// MIN = 0: no phases, each lane tests its leaf at the step after it reaches it.
+// MIN >= 1: a node phase while at least MIN lanes still want nodes (no leaf set
+// aside), or while no lane holds a leaf; otherwise a leaf phase. Waiting for every
+// lane (MIN = 1) costs closest-hit rays culling.
+template <uint32_t MIN>
+__device__ __forceinline__ bool node_phase_for(bool wants_nodes, bool active) {
+    if (MIN == 0) return wants_nodes;
+    const unsigned long long want = __ballot(wants_nodes);
+    if (MIN == 1) return want != 0ull;
+    return uint32_t(__popcll(want)) >= MIN || __ballot(active && !wants_nodes) == 0ull;
+}
+
+// traverse()'s threshold. The megakernel and the adjoint call it with the lanes whose
+// paths are still alive; on the 1080p 1M-triangle frame, 1 measured 665 ms per frame
+// for the megakernel (no phases: 689; 8: 661; 24: 714) and the best adjoint rate.
+#ifndef PRT_LOOP_PHASE_MIN
+#define PRT_LOOP_PHASE_MIN 1
+#endif
+static_assert(PRT_LOOP_PHASE_MIN >= 0 && PRT_LOOP_PHASE_MIN <= 64, "phase threshold out of range");
+
+// Closest hit (ANY = false) or occlusion test against a known best (ANY = true),
+// run to completion by each calling lane (trav_step in a loop; lanes that finish
+// leave the loop and the phases are chosen among the rest). Accepts t in
+// (tmin, best.t) or t == best.t with a lower original id: the closest hit is the
+// lexicographic minimum of (t, original id), independent of tree shape and
+// visiting order. Returns true when best changed (closest) or when an occluder was
+// found (any). The stack must be empty on entry, and is empty on return.
+template <bool ANY, bool COUNT, class StackType>
+__device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float tmin, HitRec& best, StackType& st,
+                                         Counts& cnt) {
+    TravRay r;
+    if (!trav_begin(r, o, d)) return false;  // det == 0 for every triangle
+    const float t0 = best.t;
+    const uint32_t id0 = best.id;
+    while (true) {
+        const bool node_phase = node_phase_for<PRT_LOOP_PHASE_MIN>(trav_wants_nodes(r), true);
+        const int s = trav_step<ANY, COUNT, false>(node_phase, sc.nodes, sc.tris, r, tmin, best, st, cnt);
+        if (s == kTravOccluded) return true;
+        if (s == kTravDone) break;
+    }
+    return !ANY && (best.t != t0 || best.id != id0);
 }
 
 // u, v and material of best.rec, by the intersection test that selected it

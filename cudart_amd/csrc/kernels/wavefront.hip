@@ -211,13 +211,9 @@ static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold out of range");
 
 __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__popcll(__ballot(!active))); }
 
-// Phase choice for a traversal step. MIN = 0: no phases, each lane tests its leaf
-// as soon as it reaches one. MIN >= 1: the wave runs a node phase while at least
-// MIN lanes still want nodes (no leaf set aside), or while no lane holds a leaf;
-// otherwise a leaf phase. Waiting for every lane (MIN = 1) costs closest-hit rays
-// culling; on 1080p x 128 spp over 1M triangles, 24 measured best for both kernels
-// (extend 109 -> 92 ms, shadow 54 -> 43 ms per frame against MIN = 0; 8 and 32
-// to 48 were slower).
+// Phase thresholds of the two traversal kernels (node_phase_for). On 1080p x 128
+// spp over 1M triangles, 24 measured best for both (extend 109 -> 92 ms, shadow
+// 54 -> 43 ms per frame against no phases; 8 and 32 to 48 were slower).
 #ifndef PRT_EXTEND_PHASE_MIN
 #define PRT_EXTEND_PHASE_MIN 24
 #endif
@@ -226,14 +222,6 @@ __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__
 #endif
 static_assert(PRT_EXTEND_PHASE_MIN >= 0 && PRT_EXTEND_PHASE_MIN <= 64, "phase threshold out of range");
 static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase threshold out of range");
-
-template <uint32_t MIN>
-__device__ __forceinline__ bool node_phase_for(bool wants_nodes, bool active) {
-    if (MIN == 0) return wants_nodes;
-    const unsigned long long want = __ballot(wants_nodes);
-    if (MIN == 1) return want != 0ull;
-    return uint32_t(__popcll(want)) >= MIN || __ballot(active && !wants_nodes) == 0ull;
-}
 
 // Adds the wave's counts to the counter slots; ANYHIT also books nodes and
 // triangles as the shadow (any-hit) kernel's share.
