@@ -206,6 +206,12 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
 // boxes against [tmin, tfar], and orders the children hit nearest first. Pushes the
 // farther ones (farthest first, so the nearest of them is popped next) and returns
 // the nearest, or kNoChild.
+//
+// The near and far plane of each axis are picked once per node from the sign of
+// the inverse direction, instead of a min and a max per child and axis: for i > 0,
+// q_lo <= q_hi gives t(q_lo) <= t(q_hi) (each rounding is monotonic), so the picked
+// planes are the values min / max would give. i is finite and nonzero
+// (slab_inverse), so no plane is NaN for a finite origin.
 constexpr int kNoChild = 0x7FFFFFFF;  // also the word of an empty slot
 __device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
     const bool sw = kb < ka;
@@ -229,16 +235,19 @@ __device__ __forceinline__ int visit_node(const float4* nodes, int node, float i
                 sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
     float key[4];
     int word[4] = {int(w.x), int(w.y), int(w.z), int(w.w)};
+    const bool fx = ix < 0.f, fy = iy < 0.f, fz = iz < 0.f;  // the hi plane is the near one
+    const uint32_t nqx = fx ? qxy.y : qxy.x, fqx = fx ? qxy.x : qxy.y;
+    const uint32_t nqy = fy ? qxy.w : qxy.z, fqy = fy ? qxy.z : qxy.w;
+    const uint32_t nqz = fz ? qz.y : qz.x, fqz = fz ? qz.x : qz.y;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float lx = __builtin_fmaf(qbyte(qxy.x, c), sx, org.x), hx = __builtin_fmaf(qbyte(qxy.y, c), sx, org.x);
-        const float ly = __builtin_fmaf(qbyte(qxy.z, c), sy, org.y), hy = __builtin_fmaf(qbyte(qxy.w, c), sy, org.y);
-        const float lz = __builtin_fmaf(qbyte(qz.x, c), sz, org.z), hz = __builtin_fmaf(qbyte(qz.y, c), sz, org.z);
-        const float a0 = __builtin_fmaf(lx, ix, -ox), b0 = __builtin_fmaf(hx, ix, -ox);
-        const float a1 = __builtin_fmaf(ly, iy, -oy), b1 = __builtin_fmaf(hy, iy, -oy);
-        const float a2 = __builtin_fmaf(lz, iz, -oz), b2 = __builtin_fmaf(hz, iz, -oz);
-        const float tn = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
-        const float tf = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tfar));
+        const float nx = __builtin_fmaf(qbyte(nqx, c), sx, org.x), fx_ = __builtin_fmaf(qbyte(fqx, c), sx, org.x);
+        const float ny = __builtin_fmaf(qbyte(nqy, c), sy, org.y), fy_ = __builtin_fmaf(qbyte(fqy, c), sy, org.y);
+        const float nz = __builtin_fmaf(qbyte(nqz, c), sz, org.z), fz_ = __builtin_fmaf(qbyte(fqz, c), sz, org.z);
+        const float tn = fmaxf(fmaxf(__builtin_fmaf(nx, ix, -ox), __builtin_fmaf(ny, iy, -oy)),
+                               fmaxf(__builtin_fmaf(nz, iz, -oz), tmin));
+        const float tf = fminf(fminf(__builtin_fmaf(fx_, ix, -ox), __builtin_fmaf(fy_, iy, -oy)),
+                               fminf(__builtin_fmaf(fz_, iz, -oz), tfar));
         const bool hit = tn <= tf && word[c] != kNoChild;
         key[c] = hit ? tn : __builtin_inff();
         word[c] = hit ? word[c] : kNoChild;
@@ -283,16 +292,30 @@ struct TravRay {
 };
 enum { kTravRunning = 0, kTravDone = 1, kTravOccluded = 2 };
 
-// Returns false when the ray cannot hit anything (zero direction).
+// Inverse direction component for the slab tests. A component below 2^-64 in
+// magnitude (exactly 0 happens: a diffuse sample with r1 or r2 = 0 on an
+// axis-aligned surface) is taken as +-2^-64: with 1/0 = inf, p * i - o * i is
+// inf - inf = NaN, which either drops the slab (the ray then visits most of the
+// tree) or, compared against a finite plane, culls a box the ray runs inside.
+// Tilted by 2^-64 the ray moves less than 2^-40 over t <= 2^24, far inside the
+// boxes' padding (2^-16 of the largest scene coordinate), so every hit closer
+// than 2^24 stays inside its boxes' slabs. The triangle tests use d itself.
+__device__ __forceinline__ float slab_inverse(float c) {
+    return fabsf(c) < 0x1p-64f ? copysignf(0x1p64f, c) : 1.f / c;
+}
+
+// Returns false when the ray cannot hit anything: a zero direction (det == 0 for
+// every triangle) or a NaN component (every comparison of the triangle test fails).
 __device__ __forceinline__ bool trav_begin(TravRay& r, V3 o, V3 d) {
     r.o = o;
     r.d = d;
-    r.ix = 1.f / d.x;
-    r.iy = 1.f / d.y;
-    r.iz = 1.f / d.z;
+    r.ix = slab_inverse(d.x);
+    r.iy = slab_inverse(d.y);
+    r.iz = slab_inverse(d.z);
     r.node = 0;
     r.leaf = kNoChild;
-    return !(d.x == 0.f && d.y == 0.f && d.z == 0.f);
+    const bool nan = d.x != d.x || d.y != d.y || d.z != d.z;
+    return !nan && !(d.x == 0.f && d.y == 0.f && d.z == 0.f);
 }
 
 // True for a lane that a node phase can advance without a leaf set aside.

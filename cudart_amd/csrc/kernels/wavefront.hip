@@ -223,6 +223,13 @@ __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__
 static_assert(PRT_EXTEND_PHASE_MIN >= 0 && PRT_EXTEND_PHASE_MIN <= 64, "phase threshold out of range");
 static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase threshold out of range");
 
+// wf_extend at 7 waves per SIMD (72 VGPRs, 12 B per lane spilled outside the
+// step): 93.2 -> 89.9 ms per frame. wf_shadow stays at its natural 6 (78 VGPRs):
+// forced to 7 it spills more and took 50 instead of 43 ms; 8 waves slowed both.
+#ifndef PRT_EXTEND_WAVES
+#define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(7)))
+#endif
+
 // Adds the wave's counts to the counter slots; ANYHIT also books nodes and
 // triangles as the shadow (any-hit) kernel's share.
 template <bool ANYHIT>
@@ -301,7 +308,7 @@ __device__ __forceinline__ void with_job(bool valid, uint32_t job, F&& f) {
 // ONE_SCENE: every job of the batch traverses the same scene, so the node and
 // triangle arrays are wave-uniform (scalar registers).
 template <bool COUNT, bool ONE_SCENE>
-__global__ __launch_bounds__(kBlock) void wf_extend(WfArgs a, uint32_t it) {
+__global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, uint32_t it) {
     __shared__ int s_stack[kStackLdsQueue * kBlock];
     const uint32_t q = it & 1u;
     const uint32_t n = a.ctr[kExt0 + q];
