@@ -163,8 +163,12 @@ def test_oracle_bvh_matches_brute_force(which):
     o3 = (tgt + g.normal(size=(4000, 3)) * (hi - lo).max() * 0.3).astype(np.float32)
     d3 = tgt - o3
     d3 = (d3 / np.linalg.norm(d3, axis=1, keepdims=True)).astype(np.float32)
-    o = np.concatenate([o, o2, o3])
-    d = np.concatenate([d, d2.astype(np.float32), d3])
+    # exactly axis-aligned rays (two zero direction components: 1/0 in the slab test)
+    o4 = (lo + g.random((1500, 3)) * (hi - lo)).astype(np.float32)
+    d4 = np.zeros((1500, 3), np.float32)
+    d4[np.arange(1500), g.integers(0, 3, 1500)] = g.choice([-1.0, 1.0], 1500)
+    o = np.concatenate([o, o2, o3, o4])
+    d = np.concatenate([d, d2.astype(np.float32), d3, d4])
     s = O.OracleScene(mesh.desc())
     t1, tt1, u1, v1 = s.trace(o, d)
     t2, tt2, u2, v2 = s.trace(o, d, brute=True)

@@ -49,6 +49,54 @@ def test_first_hit_spheres_bitexact(gpu):
     assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
 
 
+def _axis_camera(eye, d, up):
+    """A camera whose every primary ray is exactly d: with fov = 0 the pixel offsets
+    (right * fov * aspect * ni, up * fov * nj) vanish."""
+    cam = ca.Camera()
+    cam.eye[:] = [float(x) for x in eye]
+    cam.dir[:] = [float(x) for x in d]
+    cam.up[:] = [float(x) for x in up]
+    cam.right[:] = [float(x) for x in np.cross(d, up)]
+    cam.fov = 0.0
+    cam.aspect = 1.0
+    cam.aperture_width = 0.0
+    cam.focus_distance = 1.0
+    cam.xpixels = cam.ypixels = 1
+    return cam
+
+
+def test_first_hit_axis_aligned_rays_bitexact(gpu):
+    """Rays with exactly zero direction components (1/0 = inf in the slab test).
+    Regression: with p * inf - o * inf the slab parameter was NaN, and a box the ray
+    runs inside could be culled (or the ray walked most of the tree)."""
+    wl = W.spheres1m(8, 8, spp=1, instances=24)
+    scene = ca.Scene(wl.mesh)
+    orc = O.OracleScene(wl.mesh.desc())
+    c = O.cfg(max_depth=wl.max_depth, rr_start=wl.rr_start, seed=W.SEED)
+    grid = np.linspace(-11.0, 11.0, 7)
+    views = []
+    for a in grid:
+        for b in grid:
+            views.append(((a, 40.0, b), (0.0, -1.0, 0.0), (0.0, 0.0, -1.0)))   # down onto spheres and ground
+            views.append(((-60.0, 11.0 + a / 2, b), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)))  # along +x
+            views.append(((a, 11.0 + b / 2, 60.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0)))  # along -z
+    hits = 0
+    for eye, d, up in views:
+        cam = _axis_camera(eye, d, up)
+        r = ca.ParallelRenderer(scene, 1, 1, 1, camera=cam, max_depth=wl.max_depth, rr_start=wl.rr_start,
+                                seed=W.SEED, device=0)
+        tri, mat, t = r.first_hit(0)
+        r.close()
+        otri, omat, ot = orc.first_hit(cam, 1, 1, 0, c)
+        btri, _, bt = orc.first_hit(cam, 1, 1, 0, c, brute=True)  # O(N) scan, no boxes
+        assert np.array_equal(otri, btri) and np.array_equal(ot.view(np.uint32), bt.view(np.uint32))
+        assert np.array_equal(tri, otri), (eye, d, tri, otri)
+        assert np.array_equal(mat, omat)
+        assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+        hits += int(tri[0, 0] >= 0)
+    assert hits > len(views) // 3
+
+
 def test_radiance_cornell_matches_oracle(gpu):
     wl = W.cornell(256, 256, spp=16, max_depth=5)
     _, r, orc, c = _pair(wl, count_rays=1)
