@@ -60,9 +60,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--width", type=int, default=1920)
-    p.add_argument("--height", type=int, default=1080)
-    p.add_argument("--spp", type=int, default=128)
+    p.add_argument("--workload", choices=["spheres1m", "large"], default="spheres1m",
+                   help="spheres1m: the 1M-triangle 1080p frame (default); large: ~10M triangles with "
+                        "textures at 3840x2160, 512 spp (add --no-compare --no-adjoint to time it alone)")
+    p.add_argument("--width", type=int, default=None)
+    p.add_argument("--height", type=int, default=None)
+    p.add_argument("--spp", type=int, default=None)
     p.add_argument("--depth", type=int, default=8)
     p.add_argument("--instances", type=int, default=200)
     p.add_argument("--spp-per-launch", type=int, default=4, help="megakernel samples per launch")
@@ -78,7 +81,12 @@ def parse():
                    help="with --frames-only: launch the wavefront chunks' kernels directly, not through graphs")
     p.add_argument("--copy-probe", action="store_true",
                    help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
-    return p.parse_args()
+    a = p.parse_args()
+    big = a.workload == "large"
+    a.width = a.width or (3840 if big else 1920)
+    a.height = a.height or (2160 if big else 1080)
+    a.spp = a.spp or (512 if big else 128)
+    return a
 
 
 def copy_probe():
@@ -194,7 +202,13 @@ def main():
     from cudart_amd import workloads as W
 
     t0 = time.perf_counter()
-    wl = W.spheres1m(a.width, a.height, a.spp, a.depth, instances=a.instances)
+    if a.workload == "large":
+        wl = W.large(a.width, a.height, a.spp, a.depth)
+        described = ("large: fBm terrain (4M triangles) + 2,000 textured UV spheres (~10M triangles), "
+                     "16 1024x1024 RGBA8 textures, 64 materials, seed 0x5EED")
+    else:
+        wl = W.spheres1m(a.width, a.height, a.spp, a.depth, instances=a.instances)
+        described = "spheres1m: 200 displaced L4 icospheres + ground + 2 emissive quads, seed 0x5EED"
     scene = ca.Scene(wl.mesh)
     build_s = time.perf_counter() - t0
     info = scene.bvh_info()
@@ -249,7 +263,7 @@ def main():
         kernel_times = None
         per_ray = {"nodes": round(st.nodes_visited / rays, 2), "tris": round(st.tris_tested / rays, 2)}
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic() if wave else None
+    traffic = measured_traffic() if wave and a.workload == "spheres1m" else None  # summary is of that frame
 
     out = {
         "metric": METRIC,
@@ -264,7 +278,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "spheres1m: 200 displaced L4 icospheres + ground + 2 emissive quads, seed 0x5EED",
+        "config": {"workload": described,
                    "triangles": scene.num_triangles, "width": a.width, "height": a.height, "spp": a.spp,
                    "max_depth": a.depth, "rr_start": 3, "integrator": a.integrator},
         "ms_per_frame": round(dev_ms, 3),
