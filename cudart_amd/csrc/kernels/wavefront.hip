@@ -211,6 +211,18 @@ static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold out of range");
 
 __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__popcll(__ballot(!active))); }
 
+// A pointer that every lane of the wave holds, moved to scalar registers. Read from
+// the job table with vector loads (the compiler cannot prove that memory read-only),
+// the scene's node and triangle bases would otherwise take four VGPRs and a 64-bit
+// add per fetch; from SGPRs the fetches use the scalar base + vector offset form.
+template <class T>
+__device__ __forceinline__ const T* wave_uniform(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
+    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
+}
+
 // Phase thresholds of the two traversal kernels (node_phase_for). On 1080p x 128
 // spp over 1M triangles, 24 measured best for both (extend 109 -> 92 ms, shadow
 // 54 -> 43 ms per frame against no phases; 8 and 32 to 48 were slower).
@@ -223,11 +235,16 @@ __device__ __forceinline__ uint32_t idle_lanes(bool active) { return uint32_t(__
 static_assert(PRT_EXTEND_PHASE_MIN >= 0 && PRT_EXTEND_PHASE_MIN <= 64, "phase threshold out of range");
 static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase threshold out of range");
 
-// wf_extend at 7 waves per SIMD (72 VGPRs, 12 B per lane spilled outside the
-// step): 93.2 -> 89.9 ms per frame. wf_shadow stays at its natural 6 (78 VGPRs):
-// forced to 7 it spills more and took 50 instead of 43 ms; 8 waves slowed both.
+// Waves per SIMD of the traversal kernels. With the scene bases in SGPRs,
+// wf_extend fits 8 (63 VGPRs; the one spill is a constant reloaded only for
+// zero-direction rays) and wf_shadow 7 (72 VGPRs; its spill is reloaded only on the
+// stack's overflow path). 1080p x 128 spp over 1M triangles: 153.7 -> 151.6 ms per
+// frame against 7 and 6; before the SGPR bases, forced occupancy spilled in the loop.
+#ifndef PRT_SHADOW_WAVES
+#define PRT_SHADOW_WAVES __attribute__((amdgpu_waves_per_eu(7)))
+#endif
 #ifndef PRT_EXTEND_WAVES
-#define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(7)))
+#define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(8)))
 #endif
 
 // Adds the wave's counts to the counter slots; ANYHIT also books nodes and
@@ -326,8 +343,8 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     TravRay ray;
     ray.leaf = kNoChild;
     HitRec h;
-    const float4* nodes = ONE_SCENE ? a.jobs[0].sc.nodes : nullptr;
-    const float4* tris = ONE_SCENE ? a.jobs[0].sc.tris : nullptr;
+    const float4* nodes = ONE_SCENE ? wave_uniform(a.jobs[0].sc.nodes) : nullptr;
+    const float4* tris = ONE_SCENE ? wave_uniform(a.jobs[0].sc.tris) : nullptr;
     while (true) {
         const uint32_t idle = idle_lanes(active);
         if (idle >= kRefill) {
@@ -474,7 +491,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
 }
 
 template <bool COUNT, bool ONE_SCENE>
-__global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
+__global__ __launch_bounds__(kBlock) PRT_SHADOW_WAVES void wf_shadow(WfArgs a, uint32_t it) {
     __shared__ int s_stack[kStackLdsQueue * kBlock];
     const uint32_t q1 = (it & 1u) ^ 1u;  // the queue shade(it) filled
     const uint32_t n = a.ctr[kShadow];
@@ -487,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void wf_shadow(WfArgs a, uint32_t it) {
     TravRay ray;
     ray.leaf = kNoChild;
     HitRec occ;
-    const float4* nodes = ONE_SCENE ? a.jobs[0].sc.nodes : nullptr;
-    const float4* tris = ONE_SCENE ? a.jobs[0].sc.tris : nullptr;
+    const float4* nodes = ONE_SCENE ? wave_uniform(a.jobs[0].sc.nodes) : nullptr;
+    const float4* tris = ONE_SCENE ? wave_uniform(a.jobs[0].sc.tris) : nullptr;
     float4* dst = nullptr;  // the path's colour: next-queue entry or result slot
     V3 mask = v3(0.f, 0.f, 0.f), direct = mask;
     while (true) {
