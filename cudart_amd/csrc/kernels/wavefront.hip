@@ -291,14 +291,40 @@ __global__ void wf_advance(WfArgs a) {
     if (threadIdx.x == 0) a.ctr[kChunk] = a.ctr[kChunk] + 1;
 }
 
+// Path order within a tile. A tile that covers whole 8-row bands of an image whose
+// width is a multiple of 8 takes its pixels (for each sample) in 8x8 blocks, band by
+// band and left to right, so the 64 paths a wave takes are one block of the image
+// rather than a 64-pixel strip of one row: closer primary rays, and closer origins
+// for the shadow and extension rays that follow. Other tiles keep row order. The
+// order only decides which slot holds a (pixel, sample): every pixel still adds its
+// samples in sample order.
+#ifndef PRT_BLOCK_ORDER
+#define PRT_BLOCK_ORDER 1
+#endif
+__device__ __forceinline__ bool tile_blocked(const WfTile& t, uint32_t width) {
+    return PRT_BLOCK_ORDER && (width & 7u) == 0u && t.p0 % width == 0u && (t.p1 - t.p0) % (8u * width) == 0u;
+}
+// Position i of the tile's order -> pixel offset from p0.
+__device__ __forceinline__ uint32_t block_to_linear(uint32_t i, uint32_t width) {
+    const uint32_t blk = i >> 6, lane = i & 63u, per_band = width >> 3;
+    const uint32_t x = (blk % per_band) * 8u + (lane & 7u), y = (blk / per_band) * 8u + (lane >> 3);
+    return y * width + x;
+}
+// Pixel offset from p0 -> position in the tile's order.
+__device__ __forceinline__ uint32_t linear_to_block(uint32_t d, uint32_t width) {
+    const uint32_t x = d % width, y = d / width;
+    return (((y >> 3) * (width >> 3) + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u);
+}
+
 __global__ __launch_bounds__(256) void wf_generate(WfArgs a) {
     const WfChunk ch = a.chunks[a.ctr[kChunk]];
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < ch.paths; slot += gridDim.x * blockDim.x) {
         const WfTile t = a.tiles[find_tile(a.tiles, ch.tile0, ch.ntiles, slot, false)];
+        const WfJob& j = a.jobs[t.job];
         const uint32_t w = t.p1 - t.p0;
         const uint32_t r = slot - t.slot0;
-        const uint32_t pixel = t.p0 + r % w, sample = t.s0 + r / w;
-        const WfJob& j = a.jobs[t.job];
+        const uint32_t i = r % w, sample = t.s0 + r / w;
+        const uint32_t pixel = t.p0 + (tile_blocked(t, j.width) ? block_to_linear(i, j.width) : i);
         const int x = int(pixel % j.width), y = int(pixel / j.width);
         Rng rng{j.path.seed, j.path.view, pixel, sample};
         V3 ro, rd;
@@ -573,8 +599,9 @@ __global__ __launch_bounds__(256) void wf_resolve(WfArgs a) {
         V3 sum = v3(s.x, s.y, s.z);
         const WfJob& jb = a.jobs[t.job];
         const size_t npx = size_t(jb.width) * jb.height;
+        const uint32_t i = tile_blocked(t, jb.width) ? linear_to_block(p - t.p0, jb.width) : p - t.p0;
         for (uint32_t smp = t.s0; smp < t.s1; ++smp) {
-            const float4 r = a.result[t.slot0 + (smp - t.s0) * w + (p - t.p0)];
+            const float4 r = a.result[t.slot0 + (smp - t.s0) * w + i];
             sum = sum + v3(r);
             if (jb.samples && smp < jb.samples_cap) jb.samples[size_t(smp) * npx + p] = make_float4(r.x, r.y, r.z, 0.f);
         }
