@@ -340,11 +340,21 @@ __global__ __launch_bounds__(256) void wf_generate(WfArgs a) {
         Rng rng{j.path.seed, j.path.view, pixel, sample};
         V3 ro, rd;
         camera_ray(j.cam, x, y, rng.block(0), ro, rd);
+        // C (throughput 1, previous bsdf 0) and D (colour 0, result slot) of a new path
+        // are not written: wf_shade(0) makes them from the tile (first_bounce_cd).
         a.qa[0][slot] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(pixel | (t.job << 24)));
         a.qb[0][slot] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(sample));
-        a.qc[0][slot] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
-        a.qd[0][slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(t.slot0 + (sample - t.s0) * w + (pixel - t.p0)));
     }
+}
+
+// The C and D entries of a new path (queue position i in the first extend queue of
+// chunk ch): throughput 1, previous bsdf 0, colour 0, and the path's result slot,
+// row-major per sample within its tile.
+__device__ __forceinline__ void first_bounce_cd(const WfArgs& a, const WfChunk& ch, uint32_t i, uint32_t pixel,
+                                                uint32_t sample, float4& C, float4& D) {
+    const WfTile t = a.tiles[find_tile(a.tiles, ch.tile0, ch.ntiles, i, false)];
+    C = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
+    D = make_float4(0.f, 0.f, 0.f, __uint_as_float(t.slot0 + (sample - t.s0) * (t.p1 - t.p0) + (pixel - t.p0)));
 }
 
 // Runs f(job) with the job index made wave-uniform (scalar loads of the job
@@ -462,6 +472,9 @@ template <bool COUNT>
 __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
     const uint32_t q = it & 1u;
     const uint32_t n = a.ctr[kExt0 + q];
+    // Read once, before this kernel's queue atomics start on the counters' cache line.
+    const uint32_t chunk = __builtin_amdgcn_readfirstlane(a.ctr[kChunk]);
+    const WfChunk ch = a.chunks[chunk];
     Counts cnt = {0, 0, 0, 0, 0, 0};
     // Static, workgroup-uniform partition of the queue (shading cost varies little).
     for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
@@ -475,8 +488,12 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
         if (valid) {
             A = a.qa[q][i];
             B = a.qb[q][i];
-            Cc = a.qc[q][i];
-            D = a.qd[q][i];
+            if (it == 0) {
+                first_bounce_cd(a, ch, i, __float_as_uint(A.w) & 0xFFFFFFu, __float_as_uint(B.w), Cc, D);
+            } else {
+                Cc = a.qc[q][i];
+                D = a.qd[q][i];
+            }
             hh = a.hits[i];
             hm = a.hmat[i];
         }
