@@ -42,16 +42,27 @@ constexpr uint32_t kSpread = 64;
 constexpr int kShadeBlock = 512;
 // Bytes of queues and path state per slot: 2 queues x 64, hit 16 + 4, shadow 64, result 16.
 constexpr size_t kSlotBytes = 2 * 64 + 16 + 4 + 64 + 16;
+// Counter words per 128-byte cache line. Each queue counter and each fetch counter
+// has a line of its own: atomics on one line serialize in its L2 channel, and the
+// appends of wf_shade and the batch fetches of the traversal kernels would otherwise
+// share a few lines.
+#ifndef PRT_CTR_LINE
+#define PRT_CTR_LINE 32
+#endif
+constexpr uint32_t kCtrLine = PRT_CTR_LINE;
 enum WfCounter : uint32_t {
-    kExt0 = 0,      // ray count of queue 0
-    kExt1 = 1,      // ray count of queue 1
-    kShadow = 2,    // shadow ray count
-    kChunk = 3,     // index of the chunk being rendered
-    kPaths = 4,
-    kFetch = 8,     // per (kernel, iteration) fetch counters: 3 * kMaxIter groups of kSpread
-    kStats = kFetch + 3 * kMaxIter * 64,  // kCntNum 64-bit counters (two words each)
+    kExt0 = 0,                 // ray count of queue 0
+    kExt1 = kCtrLine,          // ray count of queue 1
+    kShadow = 2 * kCtrLine,    // shadow ray count
+    kChunk = 3 * kCtrLine,     // index of the chunk being rendered
+    kPaths = 3 * kCtrLine + 1,
+    kFetch = 8 * kCtrLine,     // per (kernel, iteration) fetch counters: 3 * kMaxIter groups of kSpread,
+                               // counter c of group g at kFetch + (g * kSpread + c) * kCtrLine
+    kStats = kFetch + 3 * kMaxIter * kSpread * kCtrLine,  // kCntNum 64-bit counters (two words each)
     kNumCounters = kStats + 2 * kCntNum,
 };
+// First fetch counter of group g (3 * iteration + kernel: 0 extend, 2 shadow).
+constexpr uint32_t fetch_group(uint32_t g) { return kFetch + g * kSpread * kCtrLine; }
 
 struct WfJob {
     SceneArgs sc;
@@ -133,7 +144,7 @@ struct SpreadFetch {
     __device__ __forceinline__ uint32_t limit(uint32_t c) const { return (nb + kSpread - 1 - c) / kSpread; }
     __device__ __forceinline__ uint32_t issue() const {
         uint32_t idx = 0xFFFFFFFFu;
-        if ((threadIdx.x & 63) == 0) idx = atomicAdd(ctr + k, 1u);
+        if ((threadIdx.x & 63) == 0) idx = atomicAdd(ctr + k * kCtrLine, 1u);
         return idx;
     }
     __device__ __forceinline__ uint32_t resolve(uint32_t raw) {
@@ -144,8 +155,8 @@ struct SpreadFetch {
             idx = 0xFFFFFFFFu;
             if ((threadIdx.x & 63) == 0) {
                 const uint32_t lim = limit(k);
-                const uint32_t seen = __hip_atomic_load(ctr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (seen < lim) idx = atomicAdd(ctr + k, 1u);
+                const uint32_t seen = __hip_atomic_load(ctr + k * kCtrLine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen < lim) idx = atomicAdd(ctr + k * kCtrLine, 1u);
                 if (idx >= lim) idx = 0xFFFFFFFFu;
             }
             idx = __shfl(idx, 0, 64);
@@ -278,9 +289,9 @@ __device__ __forceinline__ uint32_t find_tile(const WfTile* t, uint32_t first, u
 }
 
 __global__ void wf_setup(WfArgs a) {
+    for (uint32_t k = threadIdx.x; k < 3 * kMaxIter * kSpread; k += blockDim.x) a.ctr[kFetch + k * kCtrLine] = 0;
     if (threadIdx.x != 0) return;
     const uint32_t c = a.ctr[kChunk];
-    for (uint32_t k = 0; k < 3 * kMaxIter * kSpread; ++k) a.ctr[kFetch + k] = 0;
     a.ctr[kExt0] = a.chunks[c].paths;
     a.ctr[kExt1] = 0;
     a.ctr[kShadow] = 0;
@@ -384,7 +395,7 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
     LaneFeeder feed;
-    feed.start(&a.ctr[kFetch + (3 * it) * kSpread], n);
+    feed.start(&a.ctr[fetch_group(3 * it)], n);
     bool active = false;
     uint32_t i = 0;
     TravRay ray;
@@ -553,7 +564,7 @@ __global__ __launch_bounds__(kBlock) PRT_SHADOW_WAVES void wf_shadow(WfArgs a, u
     st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
     Counts cnt = {0, 0, 0, 0, 0, 0};
     LaneFeeder feed;
-    feed.start(&a.ctr[kFetch + (3 * it + 2) * kSpread], n);
+    feed.start(&a.ctr[fetch_group(3 * it + 2)], n);
     bool active = false;
     TravRay ray;
     ray.leaf = kNoChild;
