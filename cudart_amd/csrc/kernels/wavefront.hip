@@ -38,8 +38,22 @@ constexpr int kMaxIter = 64;
 // Same-address atomics serialise (about 11 ns each on MI355X), so work is handed
 // out by kSpread counters, one 64-item batch per atomic, and appends to a queue are
 // aggregated per workgroup.
-constexpr uint32_t kSpread = 64;
-constexpr int kShadeBlock = 512;
+// Fetch counters per (kernel, iteration), items per fetch atomic, and shade
+// workgroup size. With every counter on its own line, 1080p x 128 spp measured
+// within noise of 64 / 64 / 512 for spread 128, batch 128 and 1024-thread shade
+// workgroups; spread 32 and 256-thread shade workgroups were 2-4 % slower.
+#ifndef PRT_FETCH_SPREAD
+#define PRT_FETCH_SPREAD 64
+#endif
+#ifndef PRT_FETCH_BATCH
+#define PRT_FETCH_BATCH 64
+#endif
+#ifndef PRT_SHADE_BLOCK
+#define PRT_SHADE_BLOCK 512
+#endif
+constexpr uint32_t kSpread = PRT_FETCH_SPREAD;
+constexpr uint32_t kBatch = PRT_FETCH_BATCH;  // items per fetch atomic
+constexpr int kShadeBlock = PRT_SHADE_BLOCK;
 // Bytes of queues and path state per slot: 2 queues x 64, hit 16 + 4, shadow 64, result 16.
 constexpr size_t kSlotBytes = 2 * 64 + 16 + 4 + 64 + 16;
 // Counter words per 128-byte cache line. Each queue counter and each fetch counter
@@ -149,7 +163,7 @@ struct SpreadFetch {
     }
     __device__ __forceinline__ uint32_t resolve(uint32_t raw) {
         uint32_t idx = __shfl(raw, 0, 64);
-        if (idx < limit(k)) return (idx * kSpread + k) * 64u;
+        if (idx < limit(k)) return (idx * kSpread + k) * kBatch;
         for (uint32_t probe = 1; probe < kSpread; ++probe) {
             k = (k + 1) % kSpread;
             idx = 0xFFFFFFFFu;
@@ -160,14 +174,14 @@ struct SpreadFetch {
                 if (idx >= lim) idx = 0xFFFFFFFFu;
             }
             idx = __shfl(idx, 0, 64);
-            if (idx != 0xFFFFFFFFu) return (idx * kSpread + k) * 64u;
+            if (idx != 0xFFFFFFFFu) return (idx * kSpread + k) * kBatch;
         }
         return 0xFFFFFFFFu;
     }
 };
 __device__ __forceinline__ SpreadFetch make_fetch(uint32_t* ctr, uint32_t n) {
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    return SpreadFetch{ctr, wave % kSpread, (n + 63) / 64};
+    return SpreadFetch{ctr, wave % kSpread, (n + kBatch - 1) / kBatch};
 }
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -195,7 +209,7 @@ struct LaneFeeder {
         uint32_t item = kNone;
         unsigned long long m = __ballot(want);
         while (m != 0ull && base != kNone) {
-            const uint32_t size = min(64u, n - base);
+            const uint32_t size = min(kBatch, n - base);
             const uint32_t avail = size - used;
             const uint32_t rank = uint32_t(__popcll(m & below));
             if (((m >> lane) & 1ull) && rank < avail) item = base + used + rank;
@@ -902,7 +916,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     hipDeviceProp_t prop;
     WF_TRY(hipGetDeviceProperties(&prop, device));
     const int gblocks = prop.multiProcessorCount * 8;
-    const int sblocks = prop.multiProcessorCount * 2;  // 2 x 8 waves of shade per compute unit
+    const int sblocks = prop.multiProcessorCount * 1024 / kShadeBlock;  // 16 waves of shade per compute unit
 
     if (profile) {
         rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, one_scene, rs);
