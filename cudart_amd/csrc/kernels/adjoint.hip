@@ -223,16 +223,20 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
     const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
     const int lane = int(threadIdx.x);
     if (tile < a.tiles) {
-        const int x = int(tile % a.tiles_x) * 8 + (lane & 7);
-        const int y = int(tile / a.tiles_x) * 8 + (lane >> 3);
-        if (x < int(a.width) && y < int(a.height)) {
-            Stack st;
-            st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
-            Counts cnt = {0, 0, 0, 0, 0, 0};
+        Stack st;
+        st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
+        Counts cnt = {0, 0, 0, 0, 0, 0};
+        // The wave walks its 8x8 tile pixel by pixel, lane l taking samples l, l + 64, ...
+        // of each: 64 nearly equal camera rays at a time instead of 64 pixels' rays
+        // (77.6 against 81.3 ms for 512^2 x 64 samples with a pixel per lane).
+        for (int k = 0; k < kBlock; ++k) {
+            const int x = int(tile % a.tiles_x) * 8 + (k & 7);
+            const int y = int(tile / a.tiles_x) * 8 + (k >> 3);
+            if (x >= int(a.width) || y >= int(a.height)) continue;  // the same for every lane
             const size_t idx = (size_t(y) * a.width + size_t(x)) * 3;
             const D3 delta{a.norm * double(a.grad[idx]), a.norm * double(a.grad[idx + 1]),
                            a.norm * double(a.grad[idx + 2])};
-            for (uint32_t s = 0; s < a.spp; ++s) {
+            for (uint32_t s = uint32_t(lane); s < a.spp; s += kBlock) {
                 const V3 L = a.samples ? v3(a.samples[size_t(s) * a.width * a.height + idx / 3])
                                        : primal(a, x, y, s, st, cnt);
                 replay(a, x, y, s, delta, L, s_g, st, cnt);
