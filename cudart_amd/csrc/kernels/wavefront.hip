@@ -69,7 +69,6 @@ enum WfCounter : uint32_t {
     kExt1 = kCtrLine,          // ray count of queue 1
     kShadow = 2 * kCtrLine,    // shadow ray count
     kChunk = 3 * kCtrLine,     // index of the chunk being rendered
-    kPaths = 3 * kCtrLine + 1,
     kFetch = 8 * kCtrLine,     // per (kernel, iteration) fetch counters: 3 * kMaxIter groups of kSpread,
                                // counter c of group g at kFetch + (g * kSpread + c) * kCtrLine
     kStats = kFetch + 3 * kMaxIter * kSpread * kCtrLine,  // kCntNum 64-bit counters (two words each)
@@ -309,7 +308,6 @@ __global__ void wf_setup(WfArgs a) {
     a.ctr[kExt0] = a.chunks[c].paths;
     a.ctr[kExt1] = 0;
     a.ctr[kShadow] = 0;
-    a.ctr[kPaths] = a.chunks[c].paths;
 }
 
 __global__ void wf_advance(WfArgs a) {
