@@ -1158,11 +1158,12 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     // chunks were faster at every size measured, since every chunk ends in nearly empty
     // iterations). Occupancy: timed on the trial, cut into chunks of that same size
     // (or one chunk when the trial is smaller), so each trial chunk behaves like a full
-    // one. The candidates stay near full occupancy: 24 workgroups per CU is what fits.
+    // one. The candidates stay near full occupancy: wf_extend fits 32 workgroups of one
+    // wave per CU (8 per SIMD), wf_shadow 28.
     const uint64_t limit = max_slots();
     const uint32_t full_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full, limit)));
     const uint32_t trial_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full_cap, paths)));
-    const uint32_t bpcs[] = {16, 20, 24};
+    const uint32_t bpcs[] = {20, 24, 28, 32};
     for (uint32_t bpc : bpcs) {
         double t = 1e30;
         for (int rep = 0; rep < 2 && rc == PRT_OK; ++rep) {  // best of two
