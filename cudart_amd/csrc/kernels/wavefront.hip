@@ -6,13 +6,16 @@
 //
 //   setup -> generate -> [extend -> shade -> shadow] x max_depth -> resolve -> advance
 //
+//   generate camera rays (origin, direction) of the chunk's pixel-samples, queued
+//            in groups of 64 that share a pixel (or a small pixel block)
 //   extend   closest hit of every ray in the current queue (traversal only, low
 //            register pressure, persistent waves fetching 64 rays at a time)
 //   shade    shade_vertex() of every hit; survivors are compacted into the next
-//            queue and shadow rays into the shadow queue (wave ballot + one atomic
-//            per wave), finished paths write their colour to their slot
+//            queue and shadow rays into the shadow queue (wave ballots + one atomic
+//            per workgroup and queue), finished paths write their colour to their
+//            result slot (row-major per sample within the tile)
 //   shadow   occlusion test of every shadow ray; adds mask * direct
-//   resolve  adds the slots of each pixel to its running sum in sample order
+//   resolve  adds the result slots of each pixel to its running sum in sample order
 //
 // All paths of a chunk are at bounce `it` during iteration `it`, and the vertex
 // arithmetic is the megakernel's (path.hpp), so every image is bit-identical to
@@ -36,8 +39,8 @@ namespace prt {
 
 constexpr int kMaxIter = 64;
 // Same-address atomics serialise (about 11 ns each on MI355X), so work is handed
-// out by kSpread counters, one 64-item batch per atomic, and appends to a queue are
-// aggregated per workgroup.
+// out by kSpread counters, one kBatch-item batch per atomic, and appends to a queue
+// are aggregated per workgroup.
 // Fetch counters per (kernel, iteration), items per fetch atomic, and shade
 // workgroup size. With every counter on its own line, 1080p x 128 spp measured
 // within noise of 64 / 64 / 512 for spread 128, batch 128 and 1024-thread shade
