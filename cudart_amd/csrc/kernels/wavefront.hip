@@ -412,6 +412,11 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     LaneFeeder feed;
     feed.start(&a.ctr[fetch_group(3 * it)], n);
     bool active = false;
+    // A lane whose ray has finished keeps its hit in registers until the next refill
+    // point, where every such lane writes it at once: hit_details() (a triangle load
+    // and a full intersection) then runs with at least kRefill lanes instead of the
+    // few that finish in any one step.
+    bool finished = false;
     uint32_t i = 0;
     TravRay ray;
     ray.leaf = kNoChild;
@@ -421,6 +426,12 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     while (true) {
         const uint32_t idle = idle_lanes(active);
         if (idle >= kRefill) {
+            if (finished) {
+                if (h.rec != 0xFFFFFFFFu) hit_details(tris, ray, h);
+                a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.rec));
+                a.hmat[i] = h.mat;
+                finished = false;
+            }
             if (!feed.more()) {
                 if (idle == 64) break;
             } else if (const uint32_t got = feed.take(!active); got != kNone) {
@@ -451,10 +462,8 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
         const bool node_phase = node_phase_for<PRT_EXTEND_PHASE_MIN>(active && trav_wants_nodes(ray), active);
         if (active) {
             if (trav_step<false, COUNT, true>(node_phase, nodes, tris, ray, kEps, h, st, cnt) != kTravRunning) {
-                if (h.rec != 0xFFFFFFFFu) hit_details(tris, ray, h);
-                a.hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.rec));
-                a.hmat[i] = h.mat;
                 active = false;
+                finished = true;
             }
         }
     }
@@ -588,9 +597,17 @@ __global__ __launch_bounds__(kBlock) PRT_SHADOW_WAVES void wf_shadow(WfArgs a, u
     const float4* tris = ONE_SCENE ? wave_uniform(a.jobs[0].sc.tris) : nullptr;
     float4* dst = nullptr;  // the path's colour: next-queue entry or result slot
     V3 mask = v3(0.f, 0.f, 0.f), direct = mask;
+    // As in wf_extend: a finished ray's colour update waits for the next refill point.
+    bool finished = false, lit = false;
     while (true) {
         const uint32_t idle = idle_lanes(active);
         if (idle >= kRefill) {
+            if (finished) {
+                const float4 c = *dst;
+                const V3 s = add_direct(v3(c), mask, direct, lit);
+                *dst = make_float4(s.x, s.y, s.z, c.w);
+                finished = false;
+            }
             if (!feed.more()) {
                 if (idle == 64) break;
             } else if (const uint32_t got = feed.take(!active); got != kNone) {
@@ -632,10 +649,9 @@ __global__ __launch_bounds__(kBlock) PRT_SHADOW_WAVES void wf_shadow(WfArgs a, u
         if (active) {
             const int r = trav_step<true, COUNT, true>(node_phase, nodes, tris, ray, kEps, occ, st, cnt);
             if (r != kTravRunning) {
-                const float4 c = *dst;
-                const V3 s = add_direct(v3(c), mask, direct, r == kTravDone);
-                *dst = make_float4(s.x, s.y, s.z, c.w);
                 active = false;
+                finished = true;
+                lit = r == kTravDone;
             }
         }
     }
