@@ -229,9 +229,11 @@ struct LaneFeeder {
 
 // Idle lanes of a wave that trigger a refill (the rest keep traversing meanwhile).
 // kRefill <= 64, so a wave with no active lane always refills or, with no work
-// left, leaves the loop.
+// left, leaves the loop. The finished rays' writes and the new rays' set-up run at
+// that point with these lanes only; on 1080p x 128 spp over 1M triangles, 32 took
+// 125.7 ms per frame, 16 129.0, 40 128.3 and 48 139.5.
 #ifndef PRT_REFILL_LANES
-#define PRT_REFILL_LANES 16
+#define PRT_REFILL_LANES 32
 #endif
 constexpr uint32_t kRefill = PRT_REFILL_LANES;
 static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold out of range");
