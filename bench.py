@@ -60,13 +60,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["spheres1m", "large"], default="spheres1m",
+    p.add_argument("--workload", choices=["spheres1m", "large", "cornell"], default="spheres1m",
                    help="spheres1m: the 1M-triangle 1080p frame (default); large: ~10M triangles with "
-                        "textures at 3840x2160, 512 spp (add --no-compare --no-adjoint to time it alone)")
+                        "textures at 3840x2160, 512 spp (add --no-compare --no-adjoint to time it alone); "
+                        "cornell: the Cornell box fixture, 256x256, 16 spp, depth 5")
     p.add_argument("--width", type=int, default=None)
     p.add_argument("--height", type=int, default=None)
     p.add_argument("--spp", type=int, default=None)
-    p.add_argument("--depth", type=int, default=8)
+    p.add_argument("--depth", type=int, default=None)
     p.add_argument("--instances", type=int, default=200)
     p.add_argument("--spp-per-launch", type=int, default=4, help="megakernel samples per launch")
     p.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
@@ -82,10 +83,12 @@ def parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
     a = p.parse_args()
-    big = a.workload == "large"
-    a.width = a.width or (3840 if big else 1920)
-    a.height = a.height or (2160 if big else 1080)
-    a.spp = a.spp or (512 if big else 128)
+    size = {"spheres1m": (1920, 1080, 128, 8), "large": (3840, 2160, 512, 8), "cornell": (256, 256, 16, 5)}
+    w, h, spp, depth = size[a.workload]
+    a.width = a.width or w
+    a.height = a.height or h
+    a.spp = a.spp or spp
+    a.depth = a.depth or depth
     return a
 
 
@@ -152,7 +155,9 @@ def timed_frames(r, spp, warmup, steps):
     return wall, float(np.mean(dev))
 
 
-def adjoint_leg():
+def adjoint_leg(cpu_threads: int = 0):
+    """cpu_threads > 0: also time the oracle's adjoint (primal + replay of every
+    pixel-sample, fp32) on 2 samples per pixel of the same image and target."""
     import cudart_amd as ca
     from cudart_amd import workloads as W
 
@@ -183,10 +188,25 @@ def adjoint_leg():
         times.append(r.stats().last_render_ms)
     ms = float(np.median(times))
     r.close()
-    return {"value": round(512 * 512 * 64 / (ms * 1e-3) / 1e6, 2), "unit": "Mgrad-samples/s",
-            "ms": round(ms, 3), "config": {"workload": "cornell100k: 100k-triangle Cornell-like scene, 16 materials",
-                                           "width": 512, "height": 512, "spp_replayed": 64, "max_depth": 6,
-                                           "parameters": 4 * M}}
+    out = {"value": round(512 * 512 * 64 / (ms * 1e-3) / 1e6, 2), "unit": "Mgrad-samples/s",
+           "ms": round(ms, 3), "config": {"workload": "cornell100k: 100k-triangle Cornell-like scene, 16 materials",
+                                          "width": 512, "height": 512, "spp_replayed": 64, "max_depth": 6,
+                                          "parameters": 4 * M}}
+    if cpu_threads > 0:
+        import oracle as O
+
+        orc = O.OracleScene(wl.mesh.desc())
+        c = O.cfg(max_depth=wl.max_depth, rr_start=wl.rr_start, seed=W.SEED)
+        spp_cpu = 2
+        t0 = time.perf_counter()
+        orc.adjoint(wl.camera, 512, 512, spp_cpu, c, image, target, threads=cpu_threads)
+        dt = time.perf_counter() - t0
+        orc.close()
+        out["cpu_baseline"] = {"value": round(512 * 512 * spp_cpu / dt / 1e6, 3), "unit": "Mgrad-samples/s",
+                               "cores": cpu_threads, "kind": "port",
+                               "sample": f"the oracle's adjoint (primal and replay of every pixel-sample, fp32) on "
+                                         f"{spp_cpu} samples per pixel of the same 512x512 image: {dt:.1f} s"}
+    return out
 
 
 def main():
@@ -202,7 +222,10 @@ def main():
     from cudart_amd import workloads as W
 
     t0 = time.perf_counter()
-    if a.workload == "large":
+    if a.workload == "cornell":
+        wl = W.cornell(a.width, a.height, a.spp, a.depth)
+        described = "cornell: meshes/cornell.obj (36 triangles), ceiling quad emissive, seed 0x5EED"
+    elif a.workload == "large":
         wl = W.large(a.width, a.height, a.spp, a.depth)
         described = ("large: fBm terrain (4M triangles) + 2,000 textured UV spheres (~10M triangles), "
                      "16 1024x1024 RGBA8 textures, 64 materials, seed 0x5EED")
@@ -216,7 +239,7 @@ def main():
     wave = a.integrator == "wavefront"
     integ = ca.INTEGRATOR_WAVEFRONT if wave else ca.INTEGRATOR_MEGAKERNEL
     per = max(1, a.spp_per_launch)
-    base = dict(camera=wl.camera, max_depth=a.depth, seed=W.SEED, spp_per_launch=per)
+    base = dict(camera=wl.camera, max_depth=a.depth, rr_start=wl.rr_start, seed=W.SEED, spp_per_launch=per)
 
     r = ca.ParallelRenderer(scene, a.width, a.height, a.spp, integrator=integ,
                             profile_kernels=int(a.frames_only and a.direct), **base)
@@ -304,7 +327,7 @@ def main():
                              "image_equal_to_wavefront": bool(np.array_equal(image, m_image))}
         out["wavefront_over_megakernel"] = round(value / m_value, 3)
     if not a.no_adjoint:
-        out["adjoint"] = adjoint_leg()
+        out["adjoint"] = adjoint_leg(0 if a.no_cpu else (a.threads or min(16, os.cpu_count() or 1)))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds, a.threads or min(16, os.cpu_count() or 1))
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
