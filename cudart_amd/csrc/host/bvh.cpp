@@ -14,6 +14,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <limits>
 #include <atomic>
 #include <cfloat>
 #include <cmath>
@@ -217,12 +218,86 @@ struct Emitter {
     double sah = 0.0;
     double root_area = 1.0;
 
+    // SAH-optimal collapse (when set): per build node, dist[j] = least cost of the
+    // subtree in at most j slots of its parent, split[j] = how many of them its first
+    // child takes (0: keep the node as one slot), as_leaf = the one-slot form is a leaf.
+    struct Cost {
+        float dist[5];
+        int8_t split[5];
+        bool as_leaf;
+    };
+    std::vector<Cost> cost{};
+
     bool inner(int32_t b) const { return bn[b].child[0] >= 0; }
     double area(int32_t b) const { return double(box_area(pbox[b], pbox[b] + 3)); }
+    bool as_leaf(int32_t b) const { return cost.empty() ? !inner(b) : cost[size_t(b)].as_leaf; }
 
-    // Up to four build nodes that replace inner build node b: its children, with
-    // the largest inner one opened until there are four.
+    // Least-cost collapse, bottom-up (children have larger indices than parents).
+    void plan(uint32_t total, float tri_cost, uint32_t merge_max) {
+        cost.assign(total, Cost{});
+        const float inf = std::numeric_limits<float>::infinity();
+        for (int64_t i = int64_t(total) - 1; i >= 0; --i) {
+            const BNode& n = bn[i];
+            Cost& c = cost[size_t(i)];
+            const float a = float(area(int32_t(i)) / root_area);
+            const bool leafable = !inner(int32_t(i)) || n.count <= merge_max;
+            const float leaf = leafable ? a * tri_cost * float(n.count) : inf;
+            float node = inf;
+            int8_t best = 0;
+            if (inner(int32_t(i))) {
+                const Cost& l = cost[size_t(n.child[0])];
+                const Cost& r = cost[size_t(n.child[1])];
+                for (int k = 1; k <= 3; ++k) {
+                    const float v = l.dist[k] + r.dist[4 - k];
+                    if (v < node) {
+                        node = v;
+                        best = int8_t(k);
+                    }
+                }
+                node += a;
+            }
+            c.as_leaf = leaf <= node;
+            c.dist[1] = std::min(leaf, node);
+            c.split[1] = c.as_leaf ? 0 : best;  // the node form's own split of its four slots
+            for (int j = 2; j <= 4; ++j) {
+                c.dist[j] = c.dist[j - 1];
+                c.split[j] = 0;
+                if (!inner(int32_t(i))) continue;
+                const Cost& l = cost[size_t(n.child[0])];
+                const Cost& r = cost[size_t(n.child[1])];
+                for (int k = 1; k < j; ++k) {
+                    const float v = l.dist[k] + r.dist[j - k];
+                    if (v < c.dist[j]) {
+                        c.dist[j] = v;
+                        c.split[j] = int8_t(k);
+                    }
+                }
+            }
+        }
+    }
+    // The children of b in at most j slots, by the plan.
+    void expand(int32_t b, int j, int32_t* slot, int& n) const {
+        const Cost& c = cost[size_t(b)];
+        while (j >= 2 && c.split[j] == 0) --j;
+        if (j <= 1) {
+            slot[n++] = b;
+            return;
+        }
+        const int k = c.split[j];
+        expand(bn[b].child[0], k, slot, n);
+        expand(bn[b].child[1], j - k, slot, n);
+    }
+
+    // Up to four build nodes that replace inner build node b: by the plan if there
+    // is one, else its children with the largest inner one opened until there are four.
     int gather(int32_t b, int32_t slot[4]) const {
+        if (!cost.empty()) {
+            int n = 0;
+            const int k = std::max<int>(1, cost[size_t(b)].split[1]);
+            expand(bn[b].child[0], k, slot, n);
+            expand(bn[b].child[1], 4 - k, slot, n);
+            return n;
+        }
         int n = 2;
         slot[0] = bn[b].child[0];
         slot[1] = bn[b].child[1];
@@ -239,7 +314,7 @@ struct Emitter {
     }
     int32_t child_word(int32_t b, uint32_t depth) {
         const BNode& c = bn[b];
-        if (c.child[0] < 0) {
+        if (as_leaf(b)) {
             leaves++;
             max_leaf = std::max(max_leaf, c.count);
             sah += area(b) / root_area * double(c.count);
@@ -268,6 +343,8 @@ void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout&
     if (n == 0) throw std::invalid_argument("BVH of an empty scene");
     if (n >= kMaxRecords) throw std::invalid_argument("more than 2^28 triangles");
     if (params.leaf_size < 1 || params.leaf_size > kMaxLeaf) throw std::invalid_argument("leaf size out of range");
+    if (params.merge_max < params.leaf_size || params.merge_max > kMaxLeaf)
+        throw std::invalid_argument("merged leaf size out of range");
     std::vector<Prim> prims(n);
     float abs_max = 1.f;
 #pragma omp parallel for reduction(max : abs_max) schedule(static)
@@ -341,6 +418,7 @@ void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout&
         out.nodes.assign(size_t(inner) * kNodeFloats, 0.f);
         Emitter em{bn.data(), reinterpret_cast<const float(*)[6]>(pb.data()), out.nodes};
         em.root_area = std::max(1e-30, double(box_area(&pb[0], &pb[3])));
+        if (params.collapse_sah) em.plan(total, params.tri_cost, uint32_t(params.merge_max));
         em.emit(0, 0, 0);
         out.nodes.resize(size_t(em.next) * kNodeFloats);
         out.num_nodes = em.next;

@@ -45,8 +45,23 @@ struct BvhLayout {
 #ifndef PRT_BVH_LEAF_SIZE
 #define PRT_BVH_LEAF_SIZE 2  // measured best for the four-wide layout (2: 197 ms, 3-6: 203 ms per 1080p frame)
 #endif
+#ifndef PRT_BVH_COLLAPSE_SAH
+#define PRT_BVH_COLLAPSE_SAH 1  // 1080p spheres frame: 235,118 nodes (greedy 299,869), extend + shadow 1.3% faster
+#endif
+#ifndef PRT_BVH_TRI_COST
+#define PRT_BVH_TRI_COST 1.0f
+#endif
+#ifndef PRT_BVH_MERGE_MAX
+#define PRT_BVH_MERGE_MAX 2
+#endif
 struct BvhParams {
     int leaf_size = PRT_BVH_LEAF_SIZE;  // SAH leaves hold at most this many triangles
+    // Four-wide collapse: 0 = open the largest-area child until four; 1 = the
+    // collapse of least SAH cost (node visit 1, triangle test tri_cost), which may
+    // also write a subtree of at most merge_max triangles as one leaf.
+    int collapse_sah = PRT_BVH_COLLAPSE_SAH;
+    float tri_cost = PRT_BVH_TRI_COST;
+    int merge_max = PRT_BVH_MERGE_MAX;
     int sah_depth_cap = 64;   // below this depth, object-median splits only
     float pad_rel = 1.0f / 128.0f;  // leaf box padding relative to its extent
 };
