@@ -120,8 +120,9 @@ void gamma_table(uint8_t lut[256]) {
 }
 
 int ensure_spill(prt_renderer* r, uint32_t threads) {
-    // Entries beyond the LDS part of the stack, for the deepest path in this BVH.
-    const uint32_t need_entries = r->scene->bvh.max_depth + 2;
+    // Entries the deepest walk of this BVH can hold: a node visit pushes at most its
+    // three farther children, once per level 0..max_depth.
+    const uint32_t need_entries = 3 * (r->scene->bvh.max_depth + 1);
     if (need_entries <= uint32_t(kStackLds)) return PRT_OK;
     size_t bytes = size_t(kStackMax - kStackLds) * threads * sizeof(int);
     if (r->spill && r->spill_threads >= threads) return PRT_OK;

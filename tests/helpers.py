@@ -99,3 +99,58 @@ def read_png_rgba(path) -> np.ndarray:
                 p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
             cur[x] = (row[x] + p) & 0xFF
     return out.astype(np.uint8).reshape(h, w, 4)
+
+
+def stacked_quads(k: int, spacing: float = 0.01):
+    """k diffuse quads over [-1, 1]^2 at z = 0, -spacing, ..., facing +z, an emissive
+    quad above them at z = 3 facing down and one below them facing up. A ray along
+    -z meets every child box at every level of the BVH, so its traversal stack
+    grows by three entries per level (the three farther children)."""
+    zb = -spacing * k - 1.0
+    ps, ns, mids = [], [], []
+    for i in range(k):
+        z = -spacing * i
+        p, n = quad([-1, -1, z], [1, -1, z], [1, 1, z], [-1, 1, z])
+        ps.append(p)
+        ns.append(n)
+        mids += [0, 0]
+    for corners in (([-0.5, -0.5, 3], [-0.5, 0.5, 3], [0.5, 0.5, 3], [0.5, -0.5, 3]),  # facing -z
+                    ([-0.5, -0.5, zb], [0.5, -0.5, zb], [0.5, 0.5, zb], [-0.5, 0.5, zb])):  # facing +z
+        p, n = quad(*corners)
+        ps.append(p)
+        ns.append(n)
+        mids += [1, 1]
+    return np.concatenate(ps), np.concatenate(ns), mids, [material(0), material(5, ka=(1, 1, 1))]
+
+
+def stack_at_first_leaf(nodes: np.ndarray, origin, direction) -> int:
+    """Traversal stack entries held when a nearest-first walk of the four-wide layout
+    (export_bvh's nodes) first reaches a leaf: each visited node pushes its hit
+    children but the nearest, and the walk goes on with the nearest. The node
+    layout: origin xyz, exponent bytes, 4 child words, then per axis the low and
+    high 8-bit planes of the four children."""
+    u = nodes.view(np.uint32)
+    o = np.asarray(origin, np.float64)
+    d = np.asarray(direction, np.float64)
+    inv = 1.0 / np.where(np.abs(d) < 2.0 ** -64, np.copysign(2.0 ** -64, d), d)  # the kernels' clamp
+    node, depth = 0, 0
+    while True:
+        org = nodes[node, 0:3].astype(np.float64)
+        step = np.array([2.0 ** (int((u[node, 3] >> (8 * a)) & 0xFF) - 127) for a in range(3)])
+        hits = []
+        for c in range(4):
+            w = int(nodes[node, 4:8].view(np.int32)[c])
+            if w == 0x7FFFFFFF:
+                continue
+            lo = np.array([(u[node, 8 + 2 * a] >> (8 * c)) & 0xFF for a in range(3)], np.float64) * step + org
+            hi = np.array([(u[node, 9 + 2 * a] >> (8 * c)) & 0xFF for a in range(3)], np.float64) * step + org
+            t0, t1 = (lo - o) * inv, (hi - o) * inv
+            tn, tf = max(np.minimum(t0, t1).max(), 0.0), np.maximum(t0, t1).min()
+            if tn <= tf:
+                hits.append((tn, w))
+        assert hits, "the walk left the tree without reaching a leaf"
+        hits.sort()
+        depth += len(hits) - 1
+        node = hits[0][1]
+        if node < 0:
+            return depth

@@ -223,3 +223,21 @@ def test_oracle_first_hit_matches_numpy_moller_trumbore():
         best = np.min(tt[ok])
         cand = np.nonzero(ok & (tt == best))[0]
         assert ids[i] == cand.min() and ts[i] == best
+
+
+def test_stacked_quads_walk_deeper_than_the_lds_stack():
+    """The scene of the wavefront's stack-spill test: 40,000 parallel quads, where a
+    ray along -z meets every child box at every level. A nearest-first walk must
+    hold more than the queue kernels' 16 LDS stack entries when it first reaches a
+    leaf, both from the camera and from just above the top quad (shadow rays to
+    the light below the stack), so those renders go through the global spill area."""
+    from tests.helpers import OwnedDesc, stack_at_first_leaf, stacked_quads
+    p, n, m, mats = stacked_quads(40000)
+    cam = ca.camera_look_at((0, 0, 2), (0, 0, 0), (0, 1, 0), 40, 64, 48)
+    od = OwnedDesc(p, n, m, mats, cam)  # keeps the arrays alive while the scene is built
+    s = ca.Scene(od.desc)
+    nodes, _, _ = s.export_bvh()
+    grid = [(x, y) for x in (-0.5, 0.0, 0.3) for y in (-0.4, 0.1, 0.6)]
+    down = [stack_at_first_leaf(nodes, (x, y, 2.0), (0.0, 0.0, -1.0)) for x, y in grid]
+    shadow = [stack_at_first_leaf(nodes, (x, y, 1e-5), (-x / 401, -y / 401, -1.0)) for x, y in grid]
+    assert min(down) > 16 and min(shadow) > 16, (down, shadow)

@@ -84,3 +84,25 @@ def test_autotune_returns_a_working_tuning(gpu):
     imgs, _ = ca.render_batch(jobs, tune=tune, max_depth=5)
     ref, _ = _render(s, a.camera, 64, 64, 4, max_depth=5)
     assert np.array_equal(_bits(imgs[0]), _bits(ref))
+
+
+def test_deep_stack_spill_equals_megakernel_and_oracle(gpu):
+    """40,000 stacked quads with a light above and one below: primary rays and the
+    shadow rays to the lower light hold more than 16 traversal stack entries (the
+    walk is checked on the CPU in test_bvh.py), so wf_extend and wf_shadow push into
+    and pop from the global spill area. The image must still be bit-identical to the
+    megakernel's (32 LDS entries, no spill here) and match the oracle."""
+    from tests.helpers import OwnedDesc, stacked_quads
+    p, n, m, mats = stacked_quads(40000)
+    cam = ca.camera_look_at((0, 0, 2), (0, 0, 0), (0, 1, 0), 40, 64, 48)
+    od = OwnedDesc(p, n, m, mats, cam)
+    scene = ca.Scene(od.desc)
+    mk, cm = _render(scene, cam, 64, 48, 4, max_depth=4)
+    wf, cw = _render(scene, cam, 64, 48, 4, max_depth=4, integrator=ca.INTEGRATOR_WAVEFRONT)
+    assert np.array_equal(_bits(mk), _bits(wf))
+    assert cm == cw and cw[2] > 0
+    orc = O.OracleScene(od.desc)
+    acc, cnt = orc.render(cam, 64, 48, 0, 4, O.cfg(max_depth=4))
+    np.testing.assert_allclose(wf, acc / np.float32(4), rtol=1e-5, atol=1e-6)
+    assert cw == tuple(int(x) for x in cnt)
+    assert float(wf.mean()) > 0.0
