@@ -131,91 +131,118 @@ struct VertexList {
     }
 };
 
-// The replay. L comes from the primal pass of the same sample.
+// The replay of one pixel-sample, advanced a vertex at a time (replay_step) so that
+// a lane whose path has ended can start the next sample while the others go on.
+// L comes from the primal pass of the same sample.
+struct Replay {
+    PathState ps;
+    Rng rng;
+    D3 rem, delta;
+    V3 L;
+    VertexList vl;
+    uint32_t prev_ties;  // channels at the maximum at the previous roulette step (0: none)
+};
+
+__device__ __forceinline__ void replay_begin(const AdjArgs& a, int x, int y, uint32_t sample, D3 delta, V3 L,
+                                             Replay& r) {
+    r.rem = d3(L);
+    r.delta = delta;
+    r.L = L;
+    r.vl = VertexList();
+    r.prev_ties = 0;
+    r.rng = Rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
+    V3 ro, rd;
+    camera_ray(a.cam, x, y, r.rng.block(0), ro, rd);
+    path_init(r.ps, ro, rd);
+}
+
+// One vertex of the replay; false once the path has ended.
 template <class StackType>
-__device__ void replay(const AdjArgs& a, int x, int y, uint32_t sample, D3 delta, V3 L, double* g, StackType& st,
-                       Counts& cnt) {
+__device__ bool replay_step(const AdjArgs& a, Replay& r, double* g, StackType& st, Counts& cnt) {
     const SceneArgs& sc = a.sc;
     const uint32_t M = a.num_materials;
     const double gain = 3.0 * double(sc.lights_area);  // d ka / d e
-    D3 rem = d3(L);
-    VertexList vl;
-    uint32_t prev_ties = 0;  // channels at the maximum at the previous roulette step (0: none)
-    Rng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
-    V3 ro, rd;
-    camera_ray(a.cam, x, y, rng.block(0), ro, rd);
-    PathState ps;
-    path_init(ps, ro, rd);
-    while (true) {
-        HitRec h;
-        h.t = kFltMax;
-        h.id = 0;
-        h.rec = 0;
-        h.mat = 0;
-        h.u = h.v = 0.f;
-        if (!traverse<false, false>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) break;  // sky: no parameter
-        const float4 mkd = sc.mats[size_t(h.mat) * 4], mka = sc.mats[size_t(h.mat) * 4 + 1];
-        if (ps.b == 0 || ps.prev == 1 || ps.prev == 4) {
-            // the emission term shade_vertex is about to add
-            const V3 e = ps.mask * v3(mka) * sc.inv_lights_area;
-            atomicAdd(&g[3 * M + h.mat], ddot(delta, d3(ps.mask)) * gain * double(sc.inv_lights_area));
-            rem = dsub(rem, d3(e));
-        }
-        bool want_nee;
-        NeeSample ns;
-        RrInfo rr;
-        const uint32_t b = ps.b;
-        const bool more = shade_vertex<false, false>(sc, a.path, rng, ps, h, &want_nee, ns, cnt, &rr);
-        const uint32_t before = vl.n;  // diffuse vertices ahead of this one
-        if (ps.prev == 0 /*DIFFUSE*/ && __float_as_int(mka.w) < 0) {
-            const double kd[3] = {mkd.x, mkd.y, mkd.z};
-            const double rc[3] = {rem.x, rem.y, rem.z};
-            const double dc[3] = {delta.x, delta.y, delta.z};
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (kd[c] != 0.0) atomicAdd(&g[3 * h.mat + c], dc[c] * rc[c] / kd[c]);
-            vl.push(h.mat);
-        }
-        if (want_nee) {
-            ShadowReq sr;
-            const bool visible = shadow_visible(sc, ns, sr, st, cnt);
-            if (visible) {
-                rem = dsub(rem, d3(ns.mask * sr.direct));
-                atomicAdd(&g[3 * M + sr.lmat], ddot(delta, d3(ns.mask)) * gain * double(sr.wf));
-            }
-            ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
-        }
-        if (rr.p >= 0.f && ps.b == b + 1) {
-            // Survived roulette step j. Everything collected after it, R_j = L - C_j,
-            // carries 1/p_j, and with S_k[c] = sum over diffuse vertices v <= k of
-            // d log kd_{m_v,c}: d log p_j = w_j . S_j - w_{j-1} . S_{j-1}. The step adds
-            // -kappa_j d log p_j, with kappa_j = delta . R_j where p_j >= 1 (the path
-            // always survives; mask /= p_j is a deterministic rescale), and where
-            // p_j < 1 kappa_j = (hess / 2) |R_j / spp|^2: the score term of the survival
-            // probability times the loss change of that decision, delta . R_j -
-            // (hess / 2) |R_j / spp|^2, less the derivative of 1/p_j.
-            const D3 R = dsub(d3(L), d3(ps.color));
-            const double kappa = rr.p >= 1.f ? ddot(delta, R) : a.half_hess * ddot(R, R);
-            if (kappa != 0.0) {
-                const D3 w = rr_weight(rr.ties), wp = rr_weight(prev_ties);
-                for (uint32_t i = 0; i < vl.n; ++i) {
-                    const uint32_t m = vl.at(i);
-                    const float4 k4 = sc.mats[size_t(m) * 4];
-                    const double kd[3] = {k4.x, k4.y, k4.z};
-                    const double cf[3] = {i < before ? wp.x - w.x : -w.x, i < before ? wp.y - w.y : -w.y,
-                                          i < before ? wp.z - w.z : -w.z};
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        if (cf[c] != 0.0 && kd[c] != 0.0) atomicAdd(&g[3 * m + c], kappa * cf[c] / kd[c]);
-                }
-            }
-            prev_ties = rr.ties;
-        }
-        if (!more) break;
+    PathState& ps = r.ps;
+    HitRec h;
+    h.t = kFltMax;
+    h.id = 0;
+    h.rec = 0;
+    h.mat = 0;
+    h.u = h.v = 0.f;
+    if (!traverse<false, false>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) return false;  // sky: no parameter
+    const float4 mkd = sc.mats[size_t(h.mat) * 4], mka = sc.mats[size_t(h.mat) * 4 + 1];
+    if (ps.b == 0 || ps.prev == 1 || ps.prev == 4) {
+        // the emission term shade_vertex is about to add
+        const V3 e = ps.mask * v3(mka) * sc.inv_lights_area;
+        atomicAdd(&g[3 * M + h.mat], ddot(r.delta, d3(ps.mask)) * gain * double(sc.inv_lights_area));
+        r.rem = dsub(r.rem, d3(e));
     }
+    bool want_nee;
+    NeeSample ns;
+    RrInfo rr;
+    const uint32_t b = ps.b;
+    const bool more = shade_vertex<false, false>(sc, a.path, r.rng, ps, h, &want_nee, ns, cnt, &rr);
+    const uint32_t before = r.vl.n;  // diffuse vertices ahead of this one
+    if (ps.prev == 0 /*DIFFUSE*/ && __float_as_int(mka.w) < 0) {
+        const double kd[3] = {mkd.x, mkd.y, mkd.z};
+        const double rc[3] = {r.rem.x, r.rem.y, r.rem.z};
+        const double dc[3] = {r.delta.x, r.delta.y, r.delta.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (kd[c] != 0.0) atomicAdd(&g[3 * h.mat + c], dc[c] * rc[c] / kd[c]);
+        r.vl.push(h.mat);
+    }
+    if (want_nee) {
+        ShadowReq sr;
+        const bool visible = shadow_visible(sc, ns, sr, st, cnt);
+        if (visible) {
+            r.rem = dsub(r.rem, d3(ns.mask * sr.direct));
+            atomicAdd(&g[3 * M + sr.lmat], ddot(r.delta, d3(ns.mask)) * gain * double(sr.wf));
+        }
+        ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
+    }
+    if (rr.p >= 0.f && ps.b == b + 1) {
+        // Survived roulette step j. Everything collected after it, R_j = L - C_j,
+        // carries 1/p_j, and with S_k[c] = sum over diffuse vertices v <= k of
+        // d log kd_{m_v,c}: d log p_j = w_j . S_j - w_{j-1} . S_{j-1}. The step adds
+        // -kappa_j d log p_j, with kappa_j = delta . R_j where p_j >= 1 (the path
+        // always survives; mask /= p_j is a deterministic rescale), and where
+        // p_j < 1 kappa_j = (hess / 2) |R_j / spp|^2: the score term of the survival
+        // probability times the loss change of that decision, delta . R_j -
+        // (hess / 2) |R_j / spp|^2, less the derivative of 1/p_j.
+        const D3 R = dsub(d3(r.L), d3(ps.color));
+        const double kappa = rr.p >= 1.f ? ddot(r.delta, R) : a.half_hess * ddot(R, R);
+        if (kappa != 0.0) {
+            const D3 w = rr_weight(rr.ties), wp = rr_weight(r.prev_ties);
+            for (uint32_t i = 0; i < r.vl.n; ++i) {
+                const uint32_t m = r.vl.at(i);
+                const float4 k4 = sc.mats[size_t(m) * 4];
+                const double kd[3] = {k4.x, k4.y, k4.z};
+                const double cf[3] = {i < before ? wp.x - w.x : -w.x, i < before ? wp.y - w.y : -w.y,
+                                      i < before ? wp.z - w.z : -w.z};
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (cf[c] != 0.0 && kd[c] != 0.0) atomicAdd(&g[3 * m + c], kappa * cf[c] / kd[c]);
+            }
+        }
+        r.prev_ties = rr.ties;
+    }
+    return more;
 }
 
-__global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
+// Occupancy of the replay kernel (empty: the compiler's choice).
+#ifndef PRT_ADJOINT_WAVES
+#define PRT_ADJOINT_WAVES
+#endif
+// Idle lanes that make the wave start new pixel-samples. On 512^2 x 64 samples of
+// the 100k-triangle scene: 1 lane 71.7 / 73.1 ms, 8: 72.7 / 74.6, 16: 72.5 / 72.9,
+// 32: 79.8 / 79.0 (two runs each).
+#ifndef PRT_ADJOINT_REFILL
+#define PRT_ADJOINT_REFILL 1
+#endif
+constexpr uint32_t kAdjRefill = PRT_ADJOINT_REFILL;
+static_assert(kAdjRefill >= 1 && kAdjRefill <= 64, "refill threshold out of range");
+__global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjArgs a) {
     __shared__ int s_stack[kStackLds * kBlock];
     __shared__ double s_g[4 * PRT_MAX_MATERIALS];
     for (uint32_t k = threadIdx.x; k < a.num_params; k += kBlock) s_g[k] = 0.0;
@@ -226,21 +253,38 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a) {
         Stack st;
         st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
         Counts cnt = {0, 0, 0, 0, 0, 0};
-        // The wave walks its 8x8 tile pixel by pixel, lane l taking samples l, l + 64, ...
-        // of each: 64 nearly equal camera rays at a time instead of 64 pixels' rays
-        // (77.6 against 81.3 ms for 512^2 x 64 samples with a pixel per lane).
-        for (int k = 0; k < kBlock; ++k) {
-            const int x = int(tile % a.tiles_x) * 8 + (k & 7);
-            const int y = int(tile / a.tiles_x) * 8 + (k >> 3);
-            if (x >= int(a.width) || y >= int(a.height)) continue;  // the same for every lane
-            const size_t idx = (size_t(y) * a.width + size_t(x)) * 3;
-            const D3 delta{a.norm * double(a.grad[idx]), a.norm * double(a.grad[idx + 1]),
-                           a.norm * double(a.grad[idx + 2])};
-            for (uint32_t s = uint32_t(lane); s < a.spp; s += kBlock) {
-                const V3 L = a.samples ? v3(a.samples[size_t(s) * a.width * a.height + idx / 3])
-                                       : primal(a, x, y, s, st, cnt);
-                replay(a, x, y, s, delta, L, s_g, st, cnt);
+        // The wave works through its tile's pixel-samples in pixel-major order (64
+        // consecutive samples of one pixel are 64 nearly equal camera rays). A lane
+        // whose path has ended takes the next item at the next vertex step, so the
+        // wave does not wait for each group's longest path (512^2 x 64 samples: 77.9
+        // ms with one sample per lane per pixel, about 72 ms refilled).
+        const uint32_t tx = (tile % a.tiles_x) * 8, ty = (tile / a.tiles_x) * 8;
+        const uint32_t cols = min(8u, a.width - tx), rows = min(8u, a.height - ty);
+        const uint32_t items = cols * rows * a.spp;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t next = 0;  // the first item not yet taken (the same in every lane)
+        bool live = false;
+        Replay r;
+        while (true) {
+            const unsigned long long idle = __ballot(!live);
+            const uint32_t nidle = uint32_t(__popcll(idle));
+            if ((nidle >= kAdjRefill || nidle == 64u) && next < items) {
+                const uint32_t j = next + uint32_t(__popcll(idle & below));
+                if (!live && j < items) {
+                    const uint32_t p = j / a.spp, s = j - p * a.spp;
+                    const int x = int(tx + p % cols), y = int(ty + p / cols);
+                    const size_t pix = size_t(y) * a.width + size_t(x);
+                    const D3 delta{a.norm * double(a.grad[pix * 3]), a.norm * double(a.grad[pix * 3 + 1]),
+                                   a.norm * double(a.grad[pix * 3 + 2])};
+                    const V3 L = a.samples ? v3(a.samples[size_t(s) * a.width * a.height + pix])
+                                           : primal(a, x, y, s, st, cnt);
+                    replay_begin(a, x, y, s, delta, L, r);
+                    live = true;
+                }
+                next += nidle;
             }
+            if (__ballot(live) == 0ull) break;
+            if (live) live = replay_step(a, r, s_g, st, cnt);
         }
     }
     __syncthreads();
