@@ -156,52 +156,56 @@ __device__ __forceinline__ void replay_begin(const AdjArgs& a, int x, int y, uin
     path_init(r.ps, ro, rd);
 }
 
-// One vertex of the replay; false once the path has ended.
-template <class StackType>
-__device__ bool replay_step(const AdjArgs& a, Replay& r, double* g, StackType& st, Counts& cnt) {
-    const SceneArgs& sc = a.sc;
-    const uint32_t M = a.num_materials;
-    const double gain = 3.0 * double(sc.lights_area);  // d ka / d e
-    PathState& ps = r.ps;
-    HitRec h;
-    h.t = kFltMax;
-    h.id = 0;
+// A lane of the replay kernel traces either its path's extension ray or the shadow
+// ray of the vertex it has just shaded; the vertex's work is split at the shadow
+// ray: replay_hit() once the extension ray's closest hit is known, replay_after_nee()
+// once the shadow ray's visibility is known (or at once when there is none).
+enum : int { kLaneIdle = 0, kLaneExt = 1, kLaneShadow = 2, kLaneExtDone = 3, kLaneShadowDone = 4 };
+
+// What replay_after_nee() needs of the vertex that replay_hit() shaded.
+struct Pending {
+    V3 mask;         // the light sample's throughput
+    V3 direct;       // the light's contribution if visible
+    float wf;        // its geometric weight
+    uint32_t lmat;   // the light triangle's material
+    RrInfo rr;
+    uint32_t b, before;  // bounce index and diffuse vertices ahead of the vertex
+    bool more, want_nee;
+};
+
+__device__ __forceinline__ void ray_reset(HitRec& h, float t, uint32_t id) {
+    h.t = t;
+    h.id = id;
     h.rec = 0;
     h.mat = 0;
     h.u = h.v = 0.f;
-    if (!traverse<false, false>(sc, ps.ro, ps.rd, kEps, h, st, cnt)) return false;  // sky: no parameter
-    const float4 mkd = sc.mats[size_t(h.mat) * 4], mka = sc.mats[size_t(h.mat) * 4 + 1];
-    if (ps.b == 0 || ps.prev == 1 || ps.prev == 4) {
-        // the emission term shade_vertex is about to add
-        const V3 e = ps.mask * v3(mka) * sc.inv_lights_area;
-        atomicAdd(&g[3 * M + h.mat], ddot(r.delta, d3(ps.mask)) * gain * double(sc.inv_lights_area));
-        r.rem = dsub(r.rem, d3(e));
-    }
-    bool want_nee;
-    NeeSample ns;
-    RrInfo rr;
-    const uint32_t b = ps.b;
-    const bool more = shade_vertex<false, false>(sc, a.path, r.rng, ps, h, &want_nee, ns, cnt, &rr);
-    const uint32_t before = r.vl.n;  // diffuse vertices ahead of this one
-    if (ps.prev == 0 /*DIFFUSE*/ && __float_as_int(mka.w) < 0) {
-        const double kd[3] = {mkd.x, mkd.y, mkd.z};
-        const double rc[3] = {r.rem.x, r.rem.y, r.rem.z};
-        const double dc[3] = {r.delta.x, r.delta.y, r.delta.z};
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (kd[c] != 0.0) atomicAdd(&g[3 * h.mat + c], dc[c] * rc[c] / kd[c]);
-        r.vl.push(h.mat);
-    }
-    if (want_nee) {
-        ShadowReq sr;
-        const bool visible = shadow_visible(sc, ns, sr, st, cnt);
+}
+
+// Starts the closest-hit traversal of the path's current ray; false when its
+// direction cannot hit any triangle (the path then ends, as at a miss).
+template <class StackType>
+__device__ __forceinline__ bool start_ext(const Replay& r, TravRay& tr, HitRec& h, StackType& st) {
+    ray_reset(h, kFltMax, 0);
+    st.sp = 0;
+    return trav_begin(tr, r.ps.ro, r.ps.rd);
+}
+
+// The light term and the roulette terms of the vertex; then the next ray. Returns
+// the lane's next mode.
+template <class StackType>
+__device__ int replay_after_nee(const AdjArgs& a, Replay& r, const Pending& pd, bool visible, TravRay& tr, HitRec& h,
+                                double* g, StackType& st) {
+    const SceneArgs& sc = a.sc;
+    const double gain = 3.0 * double(sc.lights_area);  // d ka / d e
+    PathState& ps = r.ps;
+    if (pd.want_nee) {
         if (visible) {
-            r.rem = dsub(r.rem, d3(ns.mask * sr.direct));
-            atomicAdd(&g[3 * M + sr.lmat], ddot(r.delta, d3(ns.mask)) * gain * double(sr.wf));
+            r.rem = dsub(r.rem, d3(pd.mask * pd.direct));
+            atomicAdd(&g[3 * a.num_materials + pd.lmat], ddot(r.delta, d3(pd.mask)) * gain * double(pd.wf));
         }
-        ps.color = add_direct(ps.color, ns.mask, sr.direct, visible);
+        ps.color = add_direct(ps.color, pd.mask, pd.direct, visible);
     }
-    if (rr.p >= 0.f && ps.b == b + 1) {
+    if (pd.rr.p >= 0.f && ps.b == pd.b + 1) {
         // Survived roulette step j. Everything collected after it, R_j = L - C_j,
         // carries 1/p_j, and with S_k[c] = sum over diffuse vertices v <= k of
         // d log kd_{m_v,c}: d log p_j = w_j . S_j - w_{j-1} . S_{j-1}. The step adds
@@ -211,23 +215,75 @@ __device__ bool replay_step(const AdjArgs& a, Replay& r, double* g, StackType& s
         // probability times the loss change of that decision, delta . R_j -
         // (hess / 2) |R_j / spp|^2, less the derivative of 1/p_j.
         const D3 R = dsub(d3(r.L), d3(ps.color));
-        const double kappa = rr.p >= 1.f ? ddot(r.delta, R) : a.half_hess * ddot(R, R);
+        const double kappa = pd.rr.p >= 1.f ? ddot(r.delta, R) : a.half_hess * ddot(R, R);
         if (kappa != 0.0) {
-            const D3 w = rr_weight(rr.ties), wp = rr_weight(r.prev_ties);
+            const D3 w = rr_weight(pd.rr.ties), wp = rr_weight(r.prev_ties);
             for (uint32_t i = 0; i < r.vl.n; ++i) {
                 const uint32_t m = r.vl.at(i);
                 const float4 k4 = sc.mats[size_t(m) * 4];
                 const double kd[3] = {k4.x, k4.y, k4.z};
-                const double cf[3] = {i < before ? wp.x - w.x : -w.x, i < before ? wp.y - w.y : -w.y,
-                                      i < before ? wp.z - w.z : -w.z};
+                const double cf[3] = {i < pd.before ? wp.x - w.x : -w.x, i < pd.before ? wp.y - w.y : -w.y,
+                                      i < pd.before ? wp.z - w.z : -w.z};
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                     if (cf[c] != 0.0 && kd[c] != 0.0) atomicAdd(&g[3 * m + c], kappa * cf[c] / kd[c]);
             }
         }
-        r.prev_ties = rr.ties;
+        r.prev_ties = pd.rr.ties;
     }
-    return more;
+    if (!pd.more) return kLaneIdle;
+    return start_ext(r, tr, h, st) ? kLaneExt : kLaneIdle;
+}
+
+// The vertex at the extension ray's closest hit h (hit = false: the sky, which
+// carries no parameter): emission and diffuse terms, shade_vertex and the light
+// sample, whose shadow ray the lane then traces. Returns the lane's next mode.
+template <class StackType>
+__device__ int replay_hit(const AdjArgs& a, Replay& r, HitRec& h, bool hit, Pending& pd, TravRay& tr, double* g,
+                          StackType& st, Counts& cnt) {
+    if (!hit) return kLaneIdle;
+    const SceneArgs& sc = a.sc;
+    const uint32_t M = a.num_materials;
+    const double gain = 3.0 * double(sc.lights_area);  // d ka / d e
+    PathState& ps = r.ps;
+    const float4 mkd = sc.mats[size_t(h.mat) * 4], mka = sc.mats[size_t(h.mat) * 4 + 1];
+    if (ps.b == 0 || ps.prev == 1 || ps.prev == 4) {
+        // the emission term shade_vertex is about to add
+        const V3 e = ps.mask * v3(mka) * sc.inv_lights_area;
+        atomicAdd(&g[3 * M + h.mat], ddot(r.delta, d3(ps.mask)) * gain * double(sc.inv_lights_area));
+        r.rem = dsub(r.rem, d3(e));
+    }
+    NeeSample ns;
+    pd.b = ps.b;
+    pd.more = shade_vertex<false, false>(sc, a.path, r.rng, ps, h, &pd.want_nee, ns, cnt, &pd.rr);
+    pd.before = r.vl.n;
+    if (ps.prev == 0 /*DIFFUSE*/ && __float_as_int(mka.w) < 0) {
+        const double kd[3] = {mkd.x, mkd.y, mkd.z};
+        const double rc[3] = {r.rem.x, r.rem.y, r.rem.z};
+        const double dc[3] = {r.delta.x, r.delta.y, r.delta.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (kd[c] != 0.0) atomicAdd(&g[3 * h.mat + c], dc[c] * rc[c] / kd[c]);
+        r.vl.push(h.mat);
+    }
+    if (pd.want_nee) {
+        ShadowReq sr;
+        sr.direct = v3(0.f, 0.f, 0.f);
+        const bool reach = nee_prepare<false>(sc, ns, sr, cnt);
+        pd.mask = ns.mask;
+        pd.direct = sr.direct;
+        pd.wf = sr.wf;
+        pd.lmat = sr.lmat;
+        if (reach) {
+            // The occlusion bound: the light sample itself, (t, id) as the closest hit.
+            ray_reset(h, sr.tL, sr.lid);
+            st.sp = 0;
+            if (trav_begin(tr, sr.so, sr.sd)) return kLaneShadow;
+            return replay_after_nee(a, r, pd, true, tr, h, g, st);  // no triangle can occlude it
+        }
+        return replay_after_nee(a, r, pd, false, tr, h, g, st);
+    }
+    return replay_after_nee(a, r, pd, false, tr, h, g, st);
 }
 
 // Occupancy of the replay kernel (empty: the compiler's choice).
@@ -241,6 +297,14 @@ __device__ bool replay_step(const AdjArgs& a, Replay& r, double* g, StackType& s
 #define PRT_ADJOINT_REFILL 1
 #endif
 constexpr uint32_t kAdjRefill = PRT_ADJOINT_REFILL;
+// Lanes whose traversal has ended wait until this many have (or none is still
+// traversing), then do their vertices' work together (512^2 x 64 samples of the
+// 100k-triangle scene: 16 lanes 73.8 ms, 32: 66.3, 48: 68.8; each vertex's work at
+// the step its traversal ended: 77.9).
+#ifndef PRT_ADJOINT_BATCH
+#define PRT_ADJOINT_BATCH 32
+#endif
+constexpr uint32_t kAdjBatch = PRT_ADJOINT_BATCH;
 static_assert(kAdjRefill >= 1 && kAdjRefill <= 64, "refill threshold out of range");
 __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjArgs a) {
     __shared__ int s_stack[kStackLds * kBlock];
@@ -255,22 +319,37 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
         Counts cnt = {0, 0, 0, 0, 0, 0};
         // The wave works through its tile's pixel-samples in pixel-major order (64
         // consecutive samples of one pixel are 64 nearly equal camera rays). A lane
-        // whose path has ended takes the next item at the next vertex step, so the
-        // wave does not wait for each group's longest path (512^2 x 64 samples: 77.9
-        // ms with one sample per lane per pixel, about 72 ms refilled).
+        // whose path has ended takes the next item, so the wave does not wait for
+        // each group's longest path (512^2 x 64 samples: 77.9 ms with one sample per
+        // lane per pixel and whole traversals in lockstep).
         const uint32_t tx = (tile % a.tiles_x) * 8, ty = (tile / a.tiles_x) * 8;
         const uint32_t cols = min(8u, a.width - tx), rows = min(8u, a.height - ty);
         const uint32_t items = cols * rows * a.spp;
         const unsigned long long below = (1ull << lane) - 1ull;
         uint32_t next = 0;  // the first item not yet taken (the same in every lane)
-        bool live = false;
+        int mode = kLaneIdle;
         Replay r;
+        TravRay tr;
+        tr.leaf = kNoChild;
+        HitRec h;
+        Pending pd;
+        // Each step advances every busy lane's traversal by one node or leaf phase;
+        // a lane whose traversal ends does that vertex's work and starts its next ray
+        // (shadow or extension) at once, so lanes do not wait for one another's
+        // traversals, and a lane whose path has ended takes the next pixel-sample.
+        bool found = false;  // the ended traversal's result: a hit (extension) or visibility (shadow)
         while (true) {
-            const unsigned long long idle = __ballot(!live);
+            const unsigned long long done = __ballot(mode >= kLaneExtDone);
+            if (done != 0ull && (uint32_t(__popcll(done)) >= kAdjBatch ||
+                                 __ballot(mode == kLaneExt || mode == kLaneShadow) == 0ull)) {
+                if (mode == kLaneExtDone) mode = replay_hit(a, r, h, found, pd, tr, s_g, st, cnt);
+                else if (mode == kLaneShadowDone) mode = replay_after_nee(a, r, pd, found, tr, h, s_g, st);
+            }
+            const unsigned long long idle = __ballot(mode == kLaneIdle);
             const uint32_t nidle = uint32_t(__popcll(idle));
             if ((nidle >= kAdjRefill || nidle == 64u) && next < items) {
                 const uint32_t j = next + uint32_t(__popcll(idle & below));
-                if (!live && j < items) {
+                if (mode == kLaneIdle && j < items) {
                     const uint32_t p = j / a.spp, s = j - p * a.spp;
                     const int x = int(tx + p % cols), y = int(ty + p / cols);
                     const size_t pix = size_t(y) * a.width + size_t(x);
@@ -279,12 +358,21 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
                     const V3 L = a.samples ? v3(a.samples[size_t(s) * a.width * a.height + pix])
                                            : primal(a, x, y, s, st, cnt);
                     replay_begin(a, x, y, s, delta, L, r);
-                    live = true;
+                    mode = start_ext(r, tr, h, st) ? kLaneExt : kLaneIdle;
                 }
                 next += nidle;
             }
-            if (__ballot(live) == 0ull) break;
-            if (live) live = replay_step(a, r, s_g, st, cnt);
+            if (__ballot(mode != kLaneIdle) == 0ull && next >= items) break;
+            const bool busy = mode == kLaneExt || mode == kLaneShadow;
+            const bool node_phase = node_phase_for<PRT_LOOP_PHASE_MIN>(busy && trav_wants_nodes(tr), busy);
+            if (busy) {
+                const int res = trav_step<false, false, false>(node_phase, a.sc.nodes, a.sc.tris, tr, kEps, h, st,
+                                                               cnt, mode == kLaneShadow);
+                if (res != kTravRunning) {
+                    found = mode == kLaneExt ? (h.t != kFltMax || h.id != 0u) : res != kTravOccluded;
+                    mode = mode == kLaneExt ? kLaneExtDone : kLaneShadowDone;
+                }
+            }
         }
     }
     __syncthreads();

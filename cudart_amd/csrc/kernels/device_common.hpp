@@ -323,9 +323,11 @@ __device__ __forceinline__ bool trav_wants_nodes(const TravRay& r) { return r.le
 
 // LEAN keeps only t, rec and id of the best hit (u, v and the material are not
 // written; hit_details() recomputes them from the record afterwards).
+// `any` (default ANY) may also be set per lane at run time, for kernels whose lanes
+// trace closest-hit and occlusion rays in the same steps.
 template <bool ANY, bool COUNT, bool LEAN, class StackType>
 __device__ __forceinline__ int trav_step(bool node_phase, const float4* nodes, const float4* tris, TravRay& r,
-                                         float tmin, HitRec& best, StackType& st, Counts& cnt) {
+                                         float tmin, HitRec& best, StackType& st, Counts& cnt, bool any = ANY) {
     if (node_phase) {
         if (uint32_t(r.node) < uint32_t(kNoChild)) {  // an inner node
             if (COUNT) cnt.nodes++;
@@ -348,7 +350,7 @@ __device__ __forceinline__ int trav_step(bool node_phase, const float4* nodes, c
             uint32_t id = __float_as_uint(a.w);
             if (!(t > tmin)) continue;
             if (t < best.t || (t == best.t && id < best.id)) {
-                if (ANY) {
+                if (any) {
                     st.sp = 0;
                     r.node = kNoChild;
                     return kTravOccluded;
