@@ -305,6 +305,12 @@ constexpr uint32_t kAdjRefill = PRT_ADJOINT_REFILL;
 #define PRT_ADJOINT_BATCH 32
 #endif
 constexpr uint32_t kAdjBatch = PRT_ADJOINT_BATCH;
+// Lanes still wanting nodes below which the wave's step is a leaf phase. With the
+// lanes tracing independently most of them are mid-traversal at any step (512^2 x
+// 64 samples: 1 lane 66.7 ms, 8: 55.1, 12: 54.1, 16: 52.6, 20: 52.8, 24: 53.9).
+#ifndef PRT_ADJOINT_PHASE_MIN
+#define PRT_ADJOINT_PHASE_MIN 16
+#endif
 static_assert(kAdjRefill >= 1 && kAdjRefill <= 64, "refill threshold out of range");
 __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjArgs a) {
     __shared__ int s_stack[kStackLds * kBlock];
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
             }
             if (__ballot(mode != kLaneIdle) == 0ull && next >= items) break;
             const bool busy = mode == kLaneExt || mode == kLaneShadow;
-            const bool node_phase = node_phase_for<PRT_LOOP_PHASE_MIN>(busy && trav_wants_nodes(tr), busy);
+            const bool node_phase = node_phase_for<PRT_ADJOINT_PHASE_MIN>(busy && trav_wants_nodes(tr), busy);
             if (busy) {
                 const int res = trav_step<false, false, false>(node_phase, a.sc.nodes, a.sc.tris, tr, kEps, h, st,
                                                                cnt, mode == kLaneShadow);
