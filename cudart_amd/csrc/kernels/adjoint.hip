@@ -131,9 +131,9 @@ struct VertexList {
     }
 };
 
-// The replay of one pixel-sample, advanced a vertex at a time (replay_step) so that
-// a lane whose path has ended can start the next sample while the others go on.
-// L comes from the primal pass of the same sample.
+// The replay state of one pixel-sample, advanced a vertex at a time (replay_hit,
+// replay_after_nee) so that a lane whose path has ended can start the next sample
+// while the others go on. L comes from the primal pass of the same sample.
 struct Replay {
     PathState ps;
     Rng rng;
@@ -160,6 +160,7 @@ __device__ __forceinline__ void replay_begin(const AdjArgs& a, int x, int y, uin
 // ray of the vertex it has just shaded; the vertex's work is split at the shadow
 // ray: replay_hit() once the extension ray's closest hit is known, replay_after_nee()
 // once the shadow ray's visibility is known (or at once when there is none).
+// The *Done modes hold a lane whose traversal has ended until its vertex's work runs.
 enum : int { kLaneIdle = 0, kLaneExt = 1, kLaneShadow = 2, kLaneExtDone = 3, kLaneShadowDone = 4 };
 
 // What replay_after_nee() needs of the vertex that replay_hit() shaded.
