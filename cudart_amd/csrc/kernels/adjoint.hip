@@ -387,13 +387,23 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
     for (uint32_t k = threadIdx.x; k < a.num_params; k += kBlock) out[k] = s_g[k];
 }
 
-// Sums the per-workgroup partials of every parameter in workgroup order.
-__global__ void adjoint_reduce(const double* partial, uint32_t blocks, uint32_t num_params, double* out) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= num_params) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[size_t(b) * num_params + k];
-    out[k] = s;
+// Sums the per-workgroup partials of parameter blockIdx.x in a fixed order: thread t
+// adds partials t, t + 256, t + 512, ... in turn, then the 256 sums are added in a
+// fixed pairwise tree. The result depends only on the partials, not on scheduling.
+constexpr uint32_t kReduceBlock = 256;
+__global__ __launch_bounds__(kReduceBlock) void adjoint_reduce(const double* partial, uint32_t blocks,
+                                                              uint32_t num_params, double* out) {
+    __shared__ double s[kReduceBlock];
+    const uint32_t k = blockIdx.x;
+    double v = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += kReduceBlock) v += partial[size_t(b) * num_params + k];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t w = kReduceBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = s[0];
 }
 
 }  // namespace dev
@@ -466,7 +476,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     ADJ_TRY(hipEventRecord(e0, r->stream));
     hipLaunchKernelGGL(dev::adjoint_kernel, dim3(grid), dim3(kBlock), 0, r->stream, a);
     ADJ_TRY(hipGetLastError());
-    hipLaunchKernelGGL(dev::adjoint_reduce, dim3((P + 255) / 256), dim3(256), 0, r->stream, d_part, grid, P, d_out);
+    hipLaunchKernelGGL(dev::adjoint_reduce, dim3(P), dim3(dev::kReduceBlock), 0, r->stream, d_part, grid, P, d_out);
     ADJ_TRY(hipGetLastError());
     ADJ_TRY(hipEventRecord(e1, r->stream));
     ADJ_TRY(hipEventSynchronize(e1));
