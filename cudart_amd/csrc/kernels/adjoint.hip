@@ -440,6 +440,8 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     float* d_grad = nullptr;
     double* d_part = nullptr;
     double* d_out = nullptr;
+    float4* d_samp = nullptr;  // this call's per-sample radiance, when none was kept
+    float4* d_acc = nullptr;
     std::vector<double> g(P);
     float ms = 0.f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -471,9 +473,47 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     a.partial = d_part;
     a.spill = r->spill;
     a.spill_stride = r->spill_threads;
+    // Without kept radiance, the megakernel's forward pass first writes each sample's
+    // radiance to a buffer of this call (when that fits in a quarter of the free
+    // memory), and the replay kernel only replays; otherwise each lane traces its
+    // sample's primal path before replaying it. The radiance is the same bit for
+    // bit either way, and so are the gradients.
+    if (!a.samples) {
+        size_t free_b = 0, total_b = 0;
+        const size_t need = px * (size_t(spp) + 1) * sizeof(float4);
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need <= free_b / 4) {
+            ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_samp), px * spp * sizeof(float4)));
+            ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_acc), px * sizeof(float4)));
+            ADJ_TRY(hipMemsetAsync(d_acc, 0, px * sizeof(float4), r->stream));
+        }
+    }
     ADJ_TRY(hipEventCreate(&e0));
     ADJ_TRY(hipEventCreate(&e1));
     ADJ_TRY(hipEventRecord(e0, r->stream));
+    if (d_samp) {
+        MegaArgs m;
+        m.sc = r->sc;
+        m.cam = a.cam;
+        m.path = a.path;
+        m.width = r->w;
+        m.height = r->h;
+        m.tiles_x = r->tiles_x;
+        m.tiles = r->tiles;
+        m.accum = d_acc;
+        m.samples = d_samp;
+        m.samples_cap = spp;
+        m.counters = nullptr;
+        m.spill = r->spill;
+        m.spill_stride = r->spill_threads;
+        const uint32_t per = r->cfg.spp_per_launch ? r->cfg.spp_per_launch : 1u;
+        for (uint32_t done_spp = 0; done_spp < spp;) {
+            m.s0 = done_spp;
+            m.nspp = std::min(per, spp - done_spp);
+            ADJ_TRY(launch_megakernel(m, false, r->stream));
+            done_spp += m.nspp;
+        }
+        a.samples = d_samp;
+    }
     hipLaunchKernelGGL(dev::adjoint_kernel, dim3(grid), dim3(kBlock), 0, r->stream, a);
     ADJ_TRY(hipGetLastError());
     hipLaunchKernelGGL(dev::adjoint_reduce, dim3(P), dim3(dev::kReduceBlock), 0, r->stream, d_part, grid, P, d_out);
@@ -488,7 +528,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
 done:
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    void* ptrs[] = {d_grad, d_part, d_out};
+    void* ptrs[] = {d_grad, d_part, d_out, d_samp, d_acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     return rc;
