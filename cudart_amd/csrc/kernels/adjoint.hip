@@ -473,7 +473,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     a.partial = d_part;
     a.spill = r->spill;
     a.spill_stride = r->spill_threads;
-    // Without kept radiance, the megakernel's forward pass first writes each sample's
+    // Without kept radiance, a wavefront forward pass first writes each sample's
     // radiance to a buffer of this call (when that fits in a quarter of the free
     // memory), and the replay kernel only replays; otherwise each lane traces its
     // sample's primal path before replaying it. The radiance is the same bit for
@@ -491,27 +491,8 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     ADJ_TRY(hipEventCreate(&e1));
     ADJ_TRY(hipEventRecord(e0, r->stream));
     if (d_samp) {
-        MegaArgs m;
-        m.sc = r->sc;
-        m.cam = a.cam;
-        m.path = a.path;
-        m.width = r->w;
-        m.height = r->h;
-        m.tiles_x = r->tiles_x;
-        m.tiles = r->tiles;
-        m.accum = d_acc;
-        m.samples = d_samp;
-        m.samples_cap = spp;
-        m.counters = nullptr;
-        m.spill = r->spill;
-        m.spill_stride = r->spill_threads;
-        const uint32_t per = r->cfg.spp_per_launch ? r->cfg.spp_per_launch : 1u;
-        for (uint32_t done_spp = 0; done_spp < spp;) {
-            m.s0 = done_spp;
-            m.nspp = std::min(per, spp - done_spp);
-            ADJ_TRY(launch_megakernel(m, false, r->stream));
-            done_spp += m.nspp;
-        }
+        rc = wavefront_samples(r, spp, d_acc, d_samp);
+        if (rc != PRT_OK) goto done;
         a.samples = d_samp;
     }
     hipLaunchKernelGGL(dev::adjoint_kernel, dim3(grid), dim3(kBlock), 0, r->stream, a);

@@ -47,5 +47,9 @@ CamArgs cam_args(const prt_camera& c);
 int set_device(int device);
 int ensure_spill(prt_renderer* r, uint32_t threads);
 int render_wavefront(prt_renderer* r, uint32_t n);  // wavefront.hip
+// Samples 0..spp-1 of the renderer's view into `accum` (zeroed) and each sample's
+// radiance into `samples` ([sample * pixels + pixel]), without touching the
+// renderer's own accumulation or counters.
+int wavefront_samples(prt_renderer* r, uint32_t spp, float4* accum, float4* samples);  // wavefront.hip
 void release_wavefront(prt_renderer* r);            // wavefront.hip
 }  // namespace prt

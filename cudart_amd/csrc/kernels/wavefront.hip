@@ -1028,6 +1028,30 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     return PRT_OK;
 }
 
+int wavefront_samples(prt_renderer* r, uint32_t spp, float4* accum, float4* samples) {
+    const bool had_state = r->wf != nullptr;
+    if (!r->wf) r->wf = new WavefrontState();
+    WfJob j;
+    j.sc = r->sc;
+    j.cam = cam_args(r->cam);
+    j.path = PathArgs{r->cfg.seed, r->cfg.view_id, r->cfg.max_depth, r->cfg.rr_start};
+    j.width = r->w;
+    j.height = r->h;
+    j.accum = accum;
+    j.samples = samples;
+    j.samples_cap = spp;
+    std::vector<WfJob> jobs = {j};
+    std::vector<PlanJob> pj = {PlanJob{r->w * r->h, 0, spp}};
+    const uint64_t total = uint64_t(r->w) * r->h * spp;
+    const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
+    RunStats rs;
+    const int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap,
+                                 r->cfg.wavefront_blocks_per_cu, false, &rs);
+    // A megakernel renderer does not keep the queues of a one-off forward pass.
+    if (!had_state && r->cfg.integrator != PRT_INTEGRATOR_WAVEFRONT) release_wavefront(r);
+    return rc;
+}
+
 void release_wavefront(prt_renderer* r) {
     if (!r->wf) return;
     free_state(*r->wf);
