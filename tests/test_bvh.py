@@ -14,8 +14,9 @@ def _tri_bounds(pos):
 
 
 def _check_product_layout(mesh):
-    a = mesh.arrays()
-    s = ca.Scene(mesh)
+    # a Mesh, or a tests.helpers.OwnedDesc
+    a = mesh.arrays() if isinstance(mesh, ca.Mesh) else ca.scene_arrays(mesh.desc)
+    s = ca.Scene(mesh if isinstance(mesh, ca.Mesh) else mesh.desc)
     info = s.bvh_info()
     nodes, recs, order = s.export_bvh()
     n = len(a.positions)
@@ -83,6 +84,30 @@ def test_product_bvh_cornell():
 def test_product_bvh_procedural_10k():
     info = _check_product_layout(ca.Mesh.generate("mixed", "tris=10000", 3))
     assert info.max_leaf_size <= 4
+
+
+@pytest.mark.parametrize("kind", ["gauss", "clustered", "duplicates", "strips"])
+def test_product_bvh_awkward_scenes(kind):
+    """The collapsed four-wide layout on inputs that stress the builder and the
+    collapse: tiny clustered triangles, many exact duplicates (coincident
+    centroids), long thin strips, and sizes from 1 to a few thousand triangles."""
+    from tests.helpers import OwnedDesc, material
+    rng = np.random.default_rng(["gauss", "clustered", "duplicates", "strips"].index(kind))
+    cam = ca.camera_look_at((0, 0, 5), (0, 0, 0), (0, 1, 0), 40, 8, 8)
+    for n in (1, 2, 3, 5, 9, 17, 64, 600, 2500):
+        if kind == "gauss":
+            p = rng.normal(size=(n, 3, 3))
+        elif kind == "clustered":
+            p = rng.normal(size=(n, 1, 3)) * 10 + rng.normal(size=(n, 3, 3)) * 1e-3
+        elif kind == "duplicates":
+            base = rng.normal(size=(1 + n // 50, 3, 3))
+            p = base[rng.integers(0, len(base), n)]
+        else:
+            p = rng.normal(size=(n, 3, 3)) * np.array([100.0, 1e-4, 1.0])
+        p = p.astype(np.float32)
+        nrm = np.zeros_like(p)
+        nrm[:, :, 2] = 1.0
+        _check_product_layout(OwnedDesc(p, nrm, [0] * n, [material(0)], cam))
 
 
 def test_product_bvh_single_leaf_and_degenerate():
