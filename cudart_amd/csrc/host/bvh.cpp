@@ -1,6 +1,6 @@
 // Binned-SAH BVH build (Wald 2007, "On fast construction of SAH-based bounding
-// volume hierarchies") over triangle centroids, 32 bins on all three axes, then a
-// depth-first child-pair layout for the HIP traversal.
+// volume hierarchies") over triangle centroids, 32 bins on all three axes, then
+// collapsed depth first into the four-wide quantized layout the HIP traversal reads.
 //
 // Compared with recursiveBuild (src/geometry/bvh.cpp:114-307) this keeps the same
 // idea (binned SAH, small leaves, object-median fallback) but: bins all axes, uses

@@ -1,4 +1,4 @@
-// Binned-SAH BVH build and the child-pair layout the HIP kernels traverse.
+// Binned-SAH BVH build and the four-wide quantized layout the HIP kernels traverse.
 // Replaces constructBVH / recursiveBuild / flattenBVHTree (src/geometry/bvh.cpp:65-323)
 // and the texture repack of ParallelRenderer::createTextureObjects
 // (src/renderer/parallel_renderer.cu:128-202).
@@ -20,8 +20,9 @@ namespace prt {
 // A child word >= 0 is an inner node index, kEmptyChild an unused slot. A negative
 // word is a leaf: bit 31 set, bits 28..30 = count-1 (1..8 triangles), bits 0..27 =
 // first record.
-// Collapsed from a binary SAH tree: a node takes its two children and keeps
-// opening the inner child with the largest box area until it has four.
+// Collapsed from a binary SAH tree into nodes of up to four children (BvhParams::
+// collapse_sah: the collapse of least SAH cost, or greedily opening the inner
+// child with the largest box area until there are four).
 constexpr int kNodeFloats = 16;
 constexpr uint32_t kNodeReadBytes = 64;
 constexpr int32_t kEmptyChild = 0x7FFFFFFF;

@@ -17,7 +17,7 @@ constexpr int kStackLdsQueue = PRT_QUEUE_STACK_LDS;  // LDS entries of the queue
 constexpr int kStackMax = 128;   // total entries (the rest in the global spill area)
 
 // Device view of a built scene. Layouts:
-//   nodes   4 x float4 per child-pair node (64 B)
+//   nodes   4 x float4 per four-wide node (64 B, quantized child boxes)
 //   tris    3 x float4 per record: {v1.xyz, id}{e1.xyz, material}{e2.xyz, 0} (48 B)
 //   shade   4 x float4 per record: normals, uvs, area (64 B)
 //   mats    4 x float4 per material: {kd, bsdf}{ka*3*A_L, tex}{ks, ns}{ni, dcoef, 0, 0}

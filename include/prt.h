@@ -133,7 +133,7 @@ typedef struct prt_stats {
     uint64_t primary_rays;
     uint64_t extension_rays;
     uint64_t shadow_rays;
-    uint64_t nodes_visited;   /* child-pair nodes fetched */
+    uint64_t nodes_visited;   /* four-wide nodes fetched (64 B each) */
     uint64_t tris_tested;     /* 48-byte intersection records fetched */
     uint64_t hits;            /* closest hits shaded (64-byte shading records fetched) */
     uint64_t samples;         /* pixel-samples completed */
@@ -214,15 +214,17 @@ int prt_env_load(const char *env_path, char *meshes, char *cameras, char *textur
 /* ---- scene build ----------------------------------------------------------- */
 
 /* Copies desc, builds the light list (Scene::constructLightList, src/geometry/scene.cpp:148-165),
- * the binned-SAH BVH in the child-pair layout (replaces constructBVH, src/geometry/bvh.cpp:65-323)
+ * the binned-SAH BVH in the four-wide layout (replaces constructBVH, src/geometry/bvh.cpp:65-323)
  * and packs materials (ParallelRenderer::createMaterialsData, src/renderer/parallel_renderer.cu:252-279).
  * Host-only work: no GPU is touched until a renderer is created. */
 int prt_scene_create(const prt_scene_desc *desc, prt_scene **out);
 int prt_scene_bvh_info(const prt_scene *scene, prt_bvh_info *info);
 /* Copies the built layout out, for host-side validation:
- * nodes: num_nodes*32 floats (128 B each: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
- * of the four children, then four child words: >= 0 inner node, 0x7FFFFFFF empty slot,
- * negative = leaf 0x80000000 | (count-1) << 28 | first record); tris: n*12 floats (48 B each);
+ * nodes: num_nodes*16 floats (64 B each: origin.xyz, then a word of per-axis exponent
+ * bytes e+127, four child words (>= 0 inner node, 0x7FFFFFFF empty slot, negative = leaf
+ * 0x80000000 | (count-1) << 28 | first record), then six words of 8-bit planes
+ * q_lo.x q_hi.x q_lo.y q_hi.y q_lo.z q_hi.z, byte c for child c, each plane
+ * fmaf(q, 2^e, origin), and two zero words); tris: n*12 floats (48 B each);
  * tri_order: n uint32 (original triangle id of each record). Any pointer may be NULL. */
 int prt_scene_export_bvh(const prt_scene *scene, float *nodes, float *tris, uint32_t *tri_order);
 int prt_scene_num_lights(const prt_scene *scene, uint32_t *num_lights, float *lights_area);
