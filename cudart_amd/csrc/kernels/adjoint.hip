@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -417,6 +418,12 @@ namespace {
             goto done;                                          \
         }                                                       \
     } while (0)
+// PRT_ADJOINT_PRIMAL_IN_REPLAY=1: without kept radiance, trace each sample's primal
+// path inside the replay kernel even where the per-sample buffer would fit (tests).
+bool primal_in_replay() {
+    const char* v = std::getenv("PRT_ADJOINT_PRIMAL_IN_REPLAY");
+    return v && v[0] == '1';
+}
 }  // namespace
 
 }  // namespace prt
@@ -478,7 +485,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     // memory), and the replay kernel only replays; otherwise each lane traces its
     // sample's primal path before replaying it. The radiance is the same bit for
     // bit either way, and so are the gradients.
-    if (!a.samples) {
+    if (!a.samples && !primal_in_replay()) {
         size_t free_b = 0, total_b = 0;
         const size_t need = px * (size_t(spp) + 1) * sizeof(float4);
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need <= free_b / 4) {

@@ -181,3 +181,22 @@ def test_adjoint_with_kept_samples_equals_recomputed(gpu, integrator):
     scene.set_material_params(0, kd, float(wl.mesh.materials[0].ka[0]))
     a2, e2, _ = rk.adjoint(target, 8, image=image)
     np.testing.assert_array_equal(a2, a_rec)
+
+
+def test_adjoint_primal_in_replay_equals_kept(gpu, monkeypatch):
+    """The fallback for when the per-sample buffer does not fit: each lane traces its
+    sample's primal path inside the replay kernel. Same radiance, same schedule,
+    so the gradients equal the kept-radiance ones bit for bit."""
+    wl = W.cornell100k(96, 80)
+    scene = ca.Scene(wl.mesh)
+    kw = dict(camera=wl.camera, max_depth=wl.max_depth, seed=W.SEED)
+    rk = ca.ParallelRenderer(scene, 96, 80, 8, keep_samples=1, **kw)
+    rk.render(8)
+    image = rk.radiance()
+    target = np.clip(image * 0.9 + 0.01, 0, None).astype(np.float32)
+    a_kept, e_kept, _ = rk.adjoint(target, 8, image=image)
+    rp = ca.ParallelRenderer(scene, 96, 80, 8, **kw)
+    monkeypatch.setenv("PRT_ADJOINT_PRIMAL_IN_REPLAY", "1")
+    a_in, e_in, _ = rp.adjoint(target, 8, image=image)
+    np.testing.assert_array_equal(a_kept, a_in)
+    np.testing.assert_array_equal(e_kept, e_in)
