@@ -212,8 +212,8 @@ def adjoint_leg(cpu_threads: int = 0):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
-              f"(the path would shard by sample range and tile across GPUs)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the renderer runs on one "
+              f"device (the path would shard by sample range and tile across GPUs)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
