@@ -3,6 +3,9 @@
 - the OBJ / MTL / camera-JSON / env loaders fed malformed and random inputs
   (tests/host_fuzz_loaders.cpp): every input is loaded or refused with an error
   code, with no invalid access, undefined behaviour or leak;
+- the command-line front end: random argument lists through its parser, then its
+  whole load / render / save sequence (PNG, PFM, progressive preview, refusals)
+  over a backend that returns a fixed image (tests/host_fuzz_cli.cpp);
 - the BVH builder on random scenes, leaf sizes 1 and 2 and merged leaves of up to
   8 triangles (the SAH collapse's leaves).
 """
@@ -79,3 +82,13 @@ def test_bvh_builder_under_sanitizers(tmp_path):
     subprocess.run([_gxx(), *SAN, str(HOST / "bvh.cpp"), str(src), "-o", str(exe)], check=True, capture_output=True,
                    timeout=600)
     assert _run(exe, cwd=tmp_path).strip().endswith("ok")
+
+
+def test_cli_under_sanitizers(tmp_path):
+    exe = tmp_path / "fuzz_cli"
+    srcs = [HOST / "cli.cpp", HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "image_io.cpp", HOST / "errors.cpp",
+            ROOT / "tests" / "host_fuzz_cli.cpp"]
+    subprocess.run([_gxx(), *SAN, *map(str, srcs), "-o", str(exe)], check=True, capture_output=True, timeout=600)
+    out = _run(exe, tmp_path, ROOT / "cudart_amd" / "assets" / "cornell.obj", cwd=tmp_path)
+    assert "run 0" in out
+    assert (tmp_path / "out.png").stat().st_size > 0 and (tmp_path / "out.pfm").stat().st_size > 0
