@@ -62,6 +62,11 @@ struct Ctx {
     BvhParams p;
 };
 
+// Bin of a centroid offset already scaled to [0, kBins): truncation, clamped to the
+// bins. NaN (an infinite extent times a zero scale, or non-finite input) goes to
+// bin 0: converting it to int would be undefined.
+inline int bin_of(float f) { return f > 0.f ? (f < float(kBins - 1) ? int(f) : kBins - 1) : 0; }
+
 void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth) {
     const std::vector<Prim>& P = *c.prims;
     BNode& nd = c.nodes[me];
@@ -94,7 +99,7 @@ void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth
             for (int b = 0; b < kBins; ++b) box_init(blo[b], bhi[b]);
             for (uint32_t i = start; i < end; ++i) {
                 const Prim& q = P[c.idx[i]];
-                int b = std::min(kBins - 1, int((q.c[ax] - clo[ax]) * scale));
+                int b = bin_of((q.c[ax] - clo[ax]) * scale);
                 cnt[b]++;
                 box_grow(blo[b], bhi[b], q.lo, q.hi);
             }
@@ -132,7 +137,7 @@ void build_rec(Ctx& c, uint32_t me, uint32_t start, uint32_t end, uint32_t depth
             const float scale = float(kBins) / (chi[ax] - clo[ax]);
             const float cmin = clo[ax];
             uint32_t* it = std::partition(c.idx + start, c.idx + end, [&](uint32_t t) {
-                return std::min(kBins - 1, int((P[t].c[ax] - cmin) * scale)) <= best_bin;
+                return bin_of((P[t].c[ax] - cmin) * scale) <= best_bin;
             });
             mid = uint32_t(it - c.idx);
             have_split = mid > start && mid < end;
@@ -256,7 +261,8 @@ struct Emitter {
                 }
                 node += a;
             }
-            c.as_leaf = leaf <= node;
+            // A binary leaf is always a leaf (its cost may be NaN when areas overflow).
+            c.as_leaf = !inner(int32_t(i)) || leaf <= node;
             c.dist[1] = std::min(leaf, node);
             c.split[1] = c.as_leaf ? 0 : best;  // the node form's own split of its four slots
             for (int j = 2; j <= 4; ++j) {

@@ -6,6 +6,10 @@
 - the command-line front end: random argument lists through its parser, then its
   whole load / render / save sequence (PNG, PFM, progressive preview, refusals)
   over a backend that returns a fixed image (tests/host_fuzz_cli.cpp);
+- the scene API (tests/host_fuzz_scene.cpp): random scenes of every BSDF with
+  textures, degenerate triangles and coordinates near the float range must build;
+  descriptions with a material, texture or BSDF out of range, an empty texture,
+  129 materials, no triangles or a non-finite vertex must be refused;
 - the BVH builder on random scenes, leaf sizes 1 and 2 and merged leaves of up to
   8 triangles (the SAH collapse's leaves).
 """
@@ -18,8 +22,8 @@ import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 HOST = ROOT / "cudart_amd" / "csrc" / "host"
-SAN = ["-std=c++17", "-O1", "-g", "-fopenmp", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
-       "-fno-sanitize-recover=undefined", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{HOST}"]
+SAN = ["-std=c++17", "-O1", "-g", "-fopenmp", "-fsanitize=address,undefined,float-cast-overflow",
+       "-fno-omit-frame-pointer", "-fno-sanitize-recover=all", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{HOST}"]
 
 BVH_DRIVER = r"""
 #include "bvh.hpp"
@@ -92,3 +96,11 @@ def test_cli_under_sanitizers(tmp_path):
     out = _run(exe, tmp_path, ROOT / "cudart_amd" / "assets" / "cornell.obj", cwd=tmp_path)
     assert "run 0" in out
     assert (tmp_path / "out.png").stat().st_size > 0 and (tmp_path / "out.pfm").stat().st_size > 0
+
+
+def test_scene_api_under_sanitizers(tmp_path):
+    exe = tmp_path / "fuzz_scene"
+    srcs = [HOST / "scene.cpp", HOST / "bvh.cpp", HOST / "errors.cpp", ROOT / "tests" / "host_fuzz_scene.cpp"]
+    subprocess.run([_gxx(), *SAN, *map(str, srcs), "-o", str(exe)], check=True, capture_output=True, timeout=600)
+    out = _run(exe, cwd=tmp_path)
+    assert "wrong 0" in out
