@@ -271,7 +271,10 @@ struct Builder {
         auto bucketOf = [&](const BBox& t) {
             float off = t.c[dim] - clo[dim];
             if (chi[dim] > clo[dim]) off /= (chi[dim] - clo[dim]);
-            int b = int(8 * off);
+            // int(8 * off), clamped to bucket 7; NaN (an extent overflowing to
+            // infinity) goes to bucket 0, as converting it to int is undefined.
+            if (!(off > 0.f)) return 0;
+            int b = off < 1.f ? int(8 * off) : 8;
             return b >= 8 ? 7 : b;
         };
         if (n <= 4) {
@@ -897,6 +900,8 @@ int oracle_scene_create(const prt_scene_desc* d, oracle_scene** out) {
     if (!d || !out || d->num_triangles == 0 || !d->positions || !d->normals || !d->material_ids ||
         d->num_materials == 0 || d->num_materials > PRT_MAX_MATERIALS || !d->materials)
         return PRT_ERR_INVALID_ARG;
+    for (size_t k = 0; k < size_t(d->num_triangles) * 9; ++k)
+        if (!std::isfinite(d->positions[k])) return PRT_ERR_INVALID_ARG;  // as prt_scene_create
     oracle_scene* s = new (std::nothrow) oracle_scene;
     if (!s) return PRT_ERR_OUT_OF_MEMORY;
     const uint32_t n = d->num_triangles;
