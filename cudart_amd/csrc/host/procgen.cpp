@@ -82,9 +82,15 @@ struct Params {
             kv[trim(item.substr(0, eq))] = trim(item.substr(eq + 1));
         }
     }
+    // The value of key k, or def. Not a number, not finite or beyond 1e9 in magnitude
+    // is an error (the callers convert to int).
     double num(const char* k, double def) const {
         auto it = kv.find(k);
-        return it == kv.end() ? def : std::stod(it->second);
+        if (it == kv.end()) return def;
+        const double v = std::stod(it->second);
+        if (!std::isfinite(v) || std::fabs(v) > 1e9)
+            throw std::invalid_argument(std::string("parameter ") + k + " out of range");
+        return v;
     }
 };
 

@@ -60,6 +60,30 @@ int main(int argc, char** argv) {
         char a[64], b[64], c[64];
         (void)prt_env_load(p.c_str(), a, b, c, sizeof a);
     }
-    std::printf("loaders under asan: %d loaded, %d refused\n", ok, err);
+    // Procedural scenes: small sizes, then one malformed or out-of-range value each.
+    const std::vector<std::pair<std::string, std::string>> gens = {
+        {"spheres", "instances=3;level=1;width=32;height=24"}, {"cornell-spheres", "instances=2;level=1"},
+        {"mixed", "tris=300;width=40"}, {"terrain-spheres", "grid=9;instances=3;stacks=4;textures=2;texsize=8"}};
+    const std::vector<std::string> bad = {"", ";;==;", "level=nan", "instances=1e300", "materials=-4", "tris=abc",
+                                          "ground=inf", "level=99", "materials=0", "width=-1e20", "texsize=x",
+                                          "emission=1e400", "spread", "=5", "radius=-1"};
+    int gen_ok = 0, gen_err = 0;
+    for (const auto& [kind, base] : gens)
+        for (const auto& extra : bad) {
+            prt_mesh* m = nullptr;
+            const std::string params = base + ";" + extra;
+            if (prt_mesh_generate(kind.c_str(), params.c_str(), 7, &m) == PRT_OK) {
+                prt_scene_desc desc;
+                prt_mesh_get_desc(m, &desc);
+                prt_mesh_destroy(m);
+                ++gen_ok;
+            } else {
+                ++gen_err;
+            }
+        }
+    prt_mesh* m = nullptr;
+    if (prt_mesh_generate("no-such-kind", "", 1, &m) == PRT_OK) return 1;
+    std::printf("loaders under asan: %d loaded, %d refused; generators: %d built, %d refused\n", ok, err, gen_ok,
+                gen_err);
     return 0;
 }

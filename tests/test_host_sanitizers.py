@@ -1,6 +1,7 @@
 """Host code under AddressSanitizer and UBSan, on the CPU (g++ only, no GPU code):
 
-- the OBJ / MTL / camera-JSON / env loaders fed malformed and random inputs
+- the OBJ / MTL / camera-JSON / env loaders fed malformed and random inputs, and
+  the procedural scene generators fed malformed and out-of-range parameters
   (tests/host_fuzz_loaders.cpp): every input is loaded or refused with an error
   code, with no invalid access, undefined behaviour or leak;
 - the command-line front end: random argument lists through its parser, then its
