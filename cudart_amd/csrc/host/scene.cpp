@@ -55,8 +55,13 @@ extern "C" int prt_scene_create(const prt_scene_desc* d, prt_scene** out) {
     s->camera = d->camera;
     for (uint32_t i = 0; i < n; ++i)
         if (d->material_ids[i] >= d->num_materials) return fail(PRT_ERR_INVALID_ARG, "material id out of range");
-    for (size_t k = 0; k < size_t(n) * 9; ++k)
+    for (size_t k = 0; k < size_t(n) * 9; ++k) {
         if (!std::isfinite(d->positions[k])) return fail(PRT_ERR_INVALID_ARG, "vertex position not finite");
+        if (!std::isfinite(d->normals[k])) return fail(PRT_ERR_INVALID_ARG, "vertex normal not finite");
+    }
+    if (d->uvs)
+        for (size_t k = 0; k < size_t(n) * 6; ++k)
+            if (!std::isfinite(d->uvs[k])) return fail(PRT_ERR_INVALID_ARG, "texture coordinate not finite");
     for (uint32_t m = 0; m < d->num_materials; ++m) {
         int t = d->materials[m].tex_kd;
         if (t != PRT_NO_TEXTURE && (t < 0 || uint32_t(t) >= d->num_textures))

@@ -900,8 +900,12 @@ int oracle_scene_create(const prt_scene_desc* d, oracle_scene** out) {
     if (!d || !out || d->num_triangles == 0 || !d->positions || !d->normals || !d->material_ids ||
         d->num_materials == 0 || d->num_materials > PRT_MAX_MATERIALS || !d->materials)
         return PRT_ERR_INVALID_ARG;
+    // Non-finite vertex data is refused, as by prt_scene_create.
     for (size_t k = 0; k < size_t(d->num_triangles) * 9; ++k)
-        if (!std::isfinite(d->positions[k])) return PRT_ERR_INVALID_ARG;  // as prt_scene_create
+        if (!std::isfinite(d->positions[k]) || !std::isfinite(d->normals[k])) return PRT_ERR_INVALID_ARG;
+    if (d->uvs)
+        for (size_t k = 0; k < size_t(d->num_triangles) * 6; ++k)
+            if (!std::isfinite(d->uvs[k])) return PRT_ERR_INVALID_ARG;
     oracle_scene* s = new (std::nothrow) oracle_scene;
     if (!s) return PRT_ERR_OUT_OF_MEMORY;
     const uint32_t n = d->num_triangles;

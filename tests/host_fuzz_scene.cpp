@@ -2,7 +2,8 @@
 // scene), built by tests/test_host_sanitizers.py with AddressSanitizer and UBSan
 // (g++, no GPU code): random valid scenes of every BSDF and with textures, and
 // descriptions that must be refused (material, texture or BSDF out of range,
-// empty textures, too many materials, no triangles, non-finite vertices), and
+// empty textures, too many materials, no triangles, non-finite positions,
+// normals or texture coordinates), and
 // degenerate triangles and coordinates near the float range, which must build.
 #include "scene.hpp"
 
@@ -114,7 +115,11 @@ int main() {
         Owned h = o;
         if (nt) h.tex[0].width = 0; else h.mats[0].bsdf = -1;    // empty texture (or negative bsdf)
         Owned i = o; i.pos[g() % i.pos.size()] = (k & 1) ? NAN : -INFINITY;  // non-finite vertex
-        for (const Owned* x : {&a, &b, &c, &e, &f, &h, &i}) {
+        Owned j = o; j.nrm[g() % j.nrm.size()] = NAN;                      // non-finite normal
+        Owned l = o;                                                       // non-finite uv
+        if (l.uv.empty()) l.uv.assign(size_t(n) * 6, 0.5f);
+        l.uv[g() % l.uv.size()] = INFINITY;
+        for (const Owned* x : {&a, &b, &c, &e, &f, &h, &i, &j, &l}) {
             if (exercise(*x) != PRT_OK) ++refused; else ++wrong;
         }
     }
