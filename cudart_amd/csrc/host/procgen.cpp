@@ -92,6 +92,12 @@ struct Params {
             throw std::invalid_argument(std::string("parameter ") + k + " out of range");
         return v;
     }
+    // A count or size: num(), which must not be negative, truncated.
+    uint32_t count(const char* k, double def) const {
+        const double v = num(k, def);
+        if (v < 0.0) throw std::invalid_argument(std::string("parameter ") + k + " is negative");
+        return uint32_t(v);
+    }
 };
 
 struct Sphere {
@@ -226,7 +232,7 @@ int gen_spheres(const Params& P, uint32_t seed, prt_mesh& m) {
     const int nmat = int(P.num("materials", 32));
     const float half = float(P.num("ground", 100.0));
     const float emission = float(P.num("emission", 15.0));
-    const uint32_t w = uint32_t(P.num("width", 1920)), h = uint32_t(P.num("height", 1080));
+    const uint32_t w = P.count("width", 1920), h = P.count("height", 1080);
     const float spread = float(P.num("spread", 10.0));
     if (instances < 0 || level < 0 || level > 7 || nmat < 1 || nmat > PRT_MAX_MATERIALS - 2)
         return fail(PRT_ERR_INVALID_ARG, "spheres: parameter out of range");
@@ -270,7 +276,7 @@ int gen_cornell_spheres(const Params& P, uint32_t seed, prt_mesh& m) {
     const int level = int(P.num("level", 4));
     const int nmat = int(P.num("materials", 16));
     const float emission = float(P.num("emission", 15.0));
-    const uint32_t w = uint32_t(P.num("width", 512)), h = uint32_t(P.num("height", 512));
+    const uint32_t w = P.count("width", 512), h = P.count("height", 512);
     if (nmat < 5 || nmat > PRT_MAX_MATERIALS || level < 0 || level > 6)
         return fail(PRT_ERR_INVALID_ARG, "cornell-spheres: parameter out of range");
     m.materials.push_back(mat_of(PRT_DIFFUSE, F3{0.6f, 0.6f, 0.6f}));
@@ -356,7 +362,7 @@ int gen_terrain_spheres(const Params& P, uint32_t seed, prt_mesh& m) {
     const int texSize = int(P.num("texsize", 1024));
     const float extent = float(P.num("extent", 200.0));
     const float emission = float(P.num("emission", 15.0));
-    const uint32_t w = uint32_t(P.num("width", 3840)), h = uint32_t(P.num("height", 2160));
+    const uint32_t w = P.count("width", 3840), h = P.count("height", 2160);
     if (grid < 2 || nmat < 4 || nmat > PRT_MAX_MATERIALS - 1 || ntex < 0 || texSize < 1 || texSize > 8192)
         return fail(PRT_ERR_INVALID_ARG, "terrain-spheres: parameter out of range");
     // Textures: smooth random colour fields, RGBA8.

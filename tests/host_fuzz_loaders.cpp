@@ -66,7 +66,7 @@ int main(int argc, char** argv) {
         {"mixed", "tris=300;width=40"}, {"terrain-spheres", "grid=9;instances=3;stacks=4;textures=2;texsize=8"}};
     const std::vector<std::string> bad = {"", ";;==;", "level=nan", "instances=1e300", "materials=-4", "tris=abc",
                                           "ground=inf", "level=99", "materials=0", "width=-1e20", "texsize=x",
-                                          "emission=1e400", "spread", "=5", "radius=-1"};
+                                          "emission=1e400", "spread", "=5", "radius=-1", "width=-5", "height=0"};
     int gen_ok = 0, gen_err = 0;
     for (const auto& [kind, base] : gens)
         for (const auto& extra : bad) {

@@ -1,4 +1,5 @@
-"""Host code under AddressSanitizer and UBSan, on the CPU (g++ only, no GPU code):
+"""Host code under AddressSanitizer and UBSan (with float-to-int overflow checks), on
+the CPU (g++ only, no GPU code):
 
 - the OBJ / MTL / camera-JSON / env loaders fed malformed and random inputs, and
   the procedural scene generators fed malformed and out-of-range parameters
@@ -10,10 +11,16 @@
 - the scene API (tests/host_fuzz_scene.cpp): random scenes of every BSDF with
   textures, degenerate triangles and coordinates near the float range must build;
   descriptions with a material, texture or BSDF out of range, an empty texture,
-  129 materials, no triangles or a non-finite vertex must be refused;
+  129 materials, no triangles or non-finite vertex data must be refused;
+- the oracle's render paths (tests/host_sanitize_oracle.cpp): generated scenes
+  with every BSDF, textures and area lights, fp32 and fp64 renders, first hits
+  (BVH and brute force) and both adjoint estimators;
 - the BVH builder on random scenes, leaf sizes 1 and 2 and merged leaves of up to
   8 triangles (the SAH collapse's leaves).
+
+The five drivers are compiled in parallel once per module.
 """
+import concurrent.futures as cf
 import os
 import shutil
 import subprocess
@@ -23,8 +30,10 @@ import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 HOST = ROOT / "cudart_amd" / "csrc" / "host"
+TESTS = ROOT / "tests"
 SAN = ["-std=c++17", "-O1", "-g", "-fopenmp", "-fsanitize=address,undefined,float-cast-overflow",
-       "-fno-omit-frame-pointer", "-fno-sanitize-recover=all", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{HOST}"]
+       "-fno-omit-frame-pointer", "-fno-sanitize-recover=all", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}",
+       f"-I{HOST}", f"-I{ROOT / 'oracle'}"]
 
 BVH_DRIVER = r"""
 #include "bvh.hpp"
@@ -52,12 +61,35 @@ int main() {
 }
 """
 
+# driver name -> sources (the bvh driver's own source is written at build time)
+DRIVERS = {
+    "loaders": [HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "errors.cpp", TESTS / "host_fuzz_loaders.cpp"],
+    "cli": [HOST / "cli.cpp", HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "image_io.cpp", HOST / "errors.cpp",
+            TESTS / "host_fuzz_cli.cpp"],
+    "scene": [HOST / "scene.cpp", HOST / "bvh.cpp", HOST / "errors.cpp", TESTS / "host_fuzz_scene.cpp"],
+    "oracle": [ROOT / "oracle" / "oracle.cpp", HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "errors.cpp",
+               TESTS / "host_sanitize_oracle.cpp"],
+    "bvh": [HOST / "bvh.cpp"],
+}
 
-def _gxx():
+
+@pytest.fixture(scope="module")
+def drivers(tmp_path_factory):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not found")
-    return gxx
+    out = tmp_path_factory.mktemp("sanitized")
+    (out / "bvh_driver.cpp").write_text(BVH_DRIVER)
+
+    def build(name):
+        srcs = DRIVERS[name] + ([out / "bvh_driver.cpp"] if name == "bvh" else [])
+        exe = out / name
+        p = subprocess.run([gxx, *SAN, *map(str, srcs), "-o", str(exe)], capture_output=True, text=True, timeout=900)
+        assert p.returncode == 0, p.stderr[-3000:]
+        return name, exe
+
+    with cf.ThreadPoolExecutor(max_workers=min(5, os.cpu_count() or 1)) as ex:
+        return dict(ex.map(build, DRIVERS))
 
 
 def _run(exe, *args, cwd):
@@ -70,38 +102,24 @@ def _run(exe, *args, cwd):
     return out
 
 
-def test_loaders_under_sanitizers(tmp_path):
-    exe = tmp_path / "fuzz_loaders"
-    srcs = [HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "errors.cpp", ROOT / "tests" / "host_fuzz_loaders.cpp"]
-    subprocess.run([_gxx(), *SAN, *map(str, srcs), "-o", str(exe)], check=True, capture_output=True, timeout=600)
-    work = tmp_path / "inputs"
-    work.mkdir()
-    out = _run(exe, work, cwd=tmp_path)
+def test_loaders_under_sanitizers(drivers, tmp_path):
+    out = _run(drivers["loaders"], tmp_path, cwd=tmp_path)
     assert "loaded" in out and "refused" in out
 
 
-def test_bvh_builder_under_sanitizers(tmp_path):
-    src = tmp_path / "bvh_driver.cpp"
-    src.write_text(BVH_DRIVER)
-    exe = tmp_path / "bvh_driver"
-    subprocess.run([_gxx(), *SAN, str(HOST / "bvh.cpp"), str(src), "-o", str(exe)], check=True, capture_output=True,
-                   timeout=600)
-    assert _run(exe, cwd=tmp_path).strip().endswith("ok")
-
-
-def test_cli_under_sanitizers(tmp_path):
-    exe = tmp_path / "fuzz_cli"
-    srcs = [HOST / "cli.cpp", HOST / "mesh_io.cpp", HOST / "procgen.cpp", HOST / "image_io.cpp", HOST / "errors.cpp",
-            ROOT / "tests" / "host_fuzz_cli.cpp"]
-    subprocess.run([_gxx(), *SAN, *map(str, srcs), "-o", str(exe)], check=True, capture_output=True, timeout=600)
-    out = _run(exe, tmp_path, ROOT / "cudart_amd" / "assets" / "cornell.obj", cwd=tmp_path)
+def test_cli_under_sanitizers(drivers, tmp_path):
+    out = _run(drivers["cli"], tmp_path, ROOT / "cudart_amd" / "assets" / "cornell.obj", cwd=tmp_path)
     assert "run 0" in out
     assert (tmp_path / "out.png").stat().st_size > 0 and (tmp_path / "out.pfm").stat().st_size > 0
 
 
-def test_scene_api_under_sanitizers(tmp_path):
-    exe = tmp_path / "fuzz_scene"
-    srcs = [HOST / "scene.cpp", HOST / "bvh.cpp", HOST / "errors.cpp", ROOT / "tests" / "host_fuzz_scene.cpp"]
-    subprocess.run([_gxx(), *SAN, *map(str, srcs), "-o", str(exe)], check=True, capture_output=True, timeout=600)
-    out = _run(exe, cwd=tmp_path)
-    assert "wrong 0" in out
+def test_scene_api_under_sanitizers(drivers, tmp_path):
+    assert "wrong 0" in _run(drivers["scene"], cwd=tmp_path)
+
+
+def test_oracle_under_sanitizers(drivers, tmp_path):
+    assert "oracle ok: 4 scenes" in _run(drivers["oracle"], cwd=tmp_path)
+
+
+def test_bvh_builder_under_sanitizers(drivers, tmp_path):
+    assert _run(drivers["bvh"], cwd=tmp_path).strip().endswith("ok")
