@@ -178,6 +178,35 @@ def test_renderer_rejects_camera_size_mismatch():
     assert e.value.code == -1
 
 
+def test_scene_refuses_non_finite_vertex_data():
+    cam = ca.camera_look_at((0, 1, 3), (0, 0, 0), (0, 1, 0), 40, 8, 8)
+    p, n = quad([-1, 0, -1], [-1, 0, 1], [1, 0, 1], [1, 0, -1])
+    for field, value in (("positions", np.nan), ("normals", np.inf), ("uvs", -np.inf)):
+        od = OwnedDesc(p, n, [0, 1], [material(0), material(5, ka=(1, 1, 1))], cam)
+        getattr(od, field)[1, 2] = value  # the arrays the description points at
+        with pytest.raises(ca.PrtError) as e:
+            ca.Scene(od.desc)
+        assert e.value.code == -1 and "not finite" in str(e.value)
+
+
+def test_without_a_device_calls_fail_cleanly():
+    """No HIP device is visible here: creating a renderer returns PRT_ERR_NO_DEVICE
+    with a message; null arguments return PRT_ERR_INVALID_ARG. Nothing crashes."""
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    wl = W.cornell(16, 16)
+    s = ca.Scene(wl.mesh)
+    with pytest.raises(ca.PrtError) as e:
+        ca.ParallelRenderer(s, 16, 16, 1, camera=wl.camera)
+    assert e.value.code == -8 and str(e.value)
+    L = _capi.lib()
+    h = C.c_void_p()
+    assert L.prt_scene_create(None, C.byref(h)) == -1 and L.prt_last_error()
+    assert L.prt_adjoint_grad(None, None, 0.0, 1, None, None) == -1
+    assert L.prt_render_spp(None, 1) == -1
+
+
 def test_png_writer_roundtrip(tmp_path):
     img = np.random.default_rng(1).integers(0, 256, (5, 7, 4), dtype=np.uint8)
     ca.write_png(tmp_path / "a.png", img)
