@@ -147,7 +147,7 @@ typedef struct prt_stats {
 
 /* BVH statistics of a built scene. */
 typedef struct prt_bvh_info {
-    uint32_t num_nodes;        /* four-wide nodes (128 B each) */
+    uint32_t num_nodes;        /* four-wide nodes (64 B each: quantized child boxes) */
     uint32_t num_leaves;
     uint32_t max_depth;
     uint32_t max_leaf_size;

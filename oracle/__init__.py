@@ -8,13 +8,16 @@ package never does.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 from cudart_amd._capi import Camera, SceneDesc
 
-LIB_PATH = Path(__file__).resolve().parent / "build" / "liboracle.so"
+# PRT_ORACLE_LIB names another build of the same sources (the mutation tests load
+# copies of the oracle with one planted mistake each through it).
+LIB_PATH = Path(os.environ.get("PRT_ORACLE_LIB") or Path(__file__).resolve().parent / "build" / "liboracle.so")
 
 _lib = None
 FP = C.POINTER(C.c_float)

@@ -519,6 +519,60 @@ V3<T> texSample(const Tex& tx, T u, T v) {
 }
 
 // ---------------------------------------------------------------------------
+// getFresnelReflectance (src/renderer/renderer.cu:464-493), with its operator
+// precedence kept: R0 = ((etai - etat)^2 / (etat + etai)) * (etat + etai), and
+// c = 1 - dot(t, n) with the unflipped normal, so Re > 1 is possible (IOR 1.5 at
+// normal incidence: R0 = 0.25, c = 2, Re = 24.25). Re == 1 marks total internal
+// reflection. samplePixel and the oracle_fresnel known-answer entry point both
+// call this one function.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Fresnel {
+    T Re, Tr;
+    V3<T> tdir;
+};
+template <class T>
+Fresnel<T> fresnelReflectance(const V3<T>& inc, const V3<T>& nrm, T ni) {
+    Fresnel<T> f{T(1), T(0), V3<T>{0, 0, 0}};
+    V3<T> n = nrm;
+    T cosi = dot(inc, nrm);
+    T etai = T(1), etat = ni;
+    if (cosi < T(0)) {
+        cosi = -cosi;
+    } else {
+        std::swap(etai, etat);
+        n = nrm * T(-1);
+    }
+    T eta = etai / etat;
+    T k = T(1) - eta * eta * (T(1) - cosi * cosi);
+    if (k < T(0)) return f;
+    f.tdir = inc * eta + n * (eta * cosi - std::sqrt(k));
+    T R0 = (etai - etat) * (etai - etat) / (etat + etai) * (etat + etai);
+    T c = T(1) - dot(f.tdir, nrm);
+    f.Re = R0 + (T(1) - R0) * c * c * c * c * c;
+    f.Tr = T(1) - f.Re;
+    return f;
+}
+
+// Direction of sampleDiffuseBSDF (renderer.cu:375-393): frame w = n,
+// u = normalize(cross(|w.x| > 0.1 ? (0,1,0) : (1,0,0), w)), v = w x u;
+// r1 = 2 pi u1 (formed in double from M_PI, then rounded), r2 = u2;
+// d = normalize(u cos r1 sqrt r2 + v sin r1 sqrt r2 + w sqrt(1 - r2)).
+// samplePixel and the oracle_diffuse_dir known-answer entry point both call it.
+template <class T>
+V3<T> diffuseDirection(const V3<T>& w, float u1f, float u2f) {
+    const T u1 = T(u1f), r2 = T(u2f);
+    T r1 = T(2 * M_PI * double(u1));
+    T r2sq = std::sqrt(r2);
+    V3<T> a = std::fabs(w.x) > T(0.1f) ? V3<T>{0, 1, 0} : V3<T>{1, 0, 0};
+    V3<T> uu = normalize(cross(a, w));
+    V3<T> vv = cross(w, uu);
+    T sr, cr;
+    sincosT<T>(r1, &sr, &cr);
+    return normalize(uu * cr * r2sq + vv * sr * r2sq + w * std::sqrt(T(1) - r2));
+}
+
+// ---------------------------------------------------------------------------
 // samplePixel (src/renderer/renderer.cu:119-262).
 // ---------------------------------------------------------------------------
 struct Counters {
@@ -616,30 +670,9 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
             }
         }
         if (currentBsdf == PRT_REFRACTIVE) {
-            // getFresnelReflectance (renderer.cu:464-493)
-            V3<T> tdir{0, 0, 0};
-            V3<T> inc = it.outputDirection, nrm = it.normal, n = nrm;
-            T cosi = dot(inc, nrm);
-            T etai = T(1), etat = T(mat.ni);
-            if (cosi < T(0)) {
-                cosi = -cosi;
-            } else {
-                std::swap(etai, etat);
-                n = nrm * T(-1);
-            }
-            T eta = etai / etat;
-            T k = T(1) - eta * eta * (T(1) - cosi * cosi);
-            T Re, Tr;
-            if (k < T(0)) {
-                Re = T(1);
-                Tr = T(0);
-            } else {
-                tdir = inc * eta + n * (eta * cosi - std::sqrt(k));
-                T R0 = (etai - etat) * (etai - etat) / (etat + etai) * (etat + etai);
-                T c = T(1) - dot(tdir, nrm);
-                Re = R0 + (T(1) - R0) * c * c * c * c * c;
-                Tr = T(1) - Re;
-            }
+            const Fresnel<T> fr = fresnelReflectance<T>(it.outputDirection, it.normal, T(mat.ni));
+            const V3<T> tdir = fr.tdir;
+            const T Re = fr.Re, Tr = fr.Tr;
             if (Re == T(1)) {
                 currentBsdf = PRT_SPECULAR;
             } else {
@@ -657,17 +690,8 @@ V3<T> samplePixel(const oracle_scene& sc, const prt_camera& cam, const oracle_cf
         }
         if (currentBsdf == PRT_DIFFUSE) {
             // sampleDiffuseBSDF (renderer.cu:375-402)
-            T u1 = T(smp.draw(blockA(b), 1)), u2 = T(smp.draw(blockA(b), 2));
-            T r1 = T(2 * M_PI * double(u1));
-            T r2 = u2;
-            T r2sq = std::sqrt(r2);
-            V3<T> w = it.normal;
-            V3<T> a = std::fabs(w.x) > T(0.1f) ? V3<T>{0, 1, 0} : V3<T>{1, 0, 0};
-            V3<T> uu = normalize(cross(a, w));
-            V3<T> vv = cross(w, uu);
-            T sr, cr;
-            sincosT<T>(r1, &sr, &cr);
-            it.inputDirection = normalize(uu * cr * r2sq + vv * sr * r2sq + w * std::sqrt(T(1) - r2));
+            const float u1 = smp.draw(blockA(b), 1), u2 = smp.draw(blockA(b), 2);
+            it.inputDirection = diffuseDirection<T>(it.normal, u1, u2);
             it.pdf = T(0.5);
             T cw = dot(it.inputDirection, it.normal);
             V3<T> kd = load3<T>(mat.kd);
@@ -1147,34 +1171,18 @@ int oracle_box_hit(const float lo[3], const float hi[3], const float o[3], const
 }
 
 int oracle_fresnel(const float wo[3], const float nn[3], float ior, float* re, float* tr, float tdir[3]) {
-    V3<float> inc = load3<float>(wo), nrm = load3<float>(nn), n = nrm, td{0, 0, 0};
-    float cosi = dot(inc, nrm), etai = 1, etat = ior;
-    if (cosi < 0.0f) cosi = -cosi;
-    else { std::swap(etai, etat); n = nrm * -1.0f; }
-    float eta = etai / etat;
-    float k = 1 - eta * eta * (1 - cosi * cosi);
-    if (k < 0) { *re = 1.0f; *tr = 0.0f; }
-    else {
-        td = inc * eta + n * (eta * cosi - std::sqrt(k));
-        float R0 = (etai - etat) * (etai - etat) / (etat + etai) * (etat + etai);
-        float c = 1.f - dot(td, nrm);
-        float Re = R0 + (1.f - R0) * c * c * c * c * c;
-        *re = Re;
-        *tr = 1.f - Re;
-    }
-    tdir[0] = td.x; tdir[1] = td.y; tdir[2] = td.z;
+    if (!wo || !nn || !re || !tr || !tdir) return PRT_ERR_INVALID_ARG;
+    const Fresnel<float> f = fresnelReflectance<float>(load3<float>(wo), load3<float>(nn), ior);
+    *re = f.Re;
+    *tr = f.Tr;
+    tdir[0] = f.tdir.x; tdir[1] = f.tdir.y; tdir[2] = f.tdir.z;
     return PRT_OK;
 }
 
 int oracle_diffuse_dir(const float nn[3], float u1, float u2, float dir[3], float* cos_weight) {
-    float r1 = float(2 * M_PI * double(u1));
-    float r2sq = std::sqrt(u2);
-    V3<float> w = load3<float>(nn);
-    V3<float> a = std::fabs(w.x) > 0.1f ? V3<float>{0, 1, 0} : V3<float>{1, 0, 0};
-    V3<float> uu = normalize(cross(a, w)), vv = cross(w, uu);
-    float s, c;
-    own_sincosf(r1, &s, &c);
-    V3<float> d = normalize(uu * c * r2sq + vv * s * r2sq + w * std::sqrt(1.f - u2));
+    if (!nn || !dir || !cos_weight) return PRT_ERR_INVALID_ARG;
+    const V3<float> w = load3<float>(nn);
+    const V3<float> d = diffuseDirection<float>(w, u1, u2);
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     *cos_weight = dot(d, w);
     return PRT_OK;

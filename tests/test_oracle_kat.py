@@ -267,3 +267,173 @@ def test_texture_sampling_identities():
     c = float(O.texture_table()[200])
     np.testing.assert_allclose(O.texture_sample(const, q), np.full((200, 3), c), rtol=1e-12)
     np.testing.assert_allclose(O.texture_sample(img, q, fp64=False), base, rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Path-level closed forms: whole renders through samplePixel, on scenes whose
+# every pixel has a value fixed by the reference's estimator
+# (src/renderer/renderer.cu:158-161 emission, :171-181 DIFFSPEC, :183-205 and
+# :464-493 REFRACTIVE, :239-243 SPECULAR, :253-259 Russian roulette). Where a
+# branch depends on a random number, the test reads that number itself from
+# Philox4x32-10 (pinned above by its published vectors) at the slot the draw
+# schedule assigns it: counter (pixel, sample, block, 0), key (seed, view);
+# block 1 + 2b = {lobe, bsdf r1, bsdf r2, light pick} and block 2 + 2b =
+# {light u, light v, roulette, spare} at bounce b; uniform = (x >> 8) 2^-24.
+# ---------------------------------------------------------------------------
+SKY = np.array([0.0, 0.025, 0.05])
+SEED = 0x5EED
+
+
+def _draw(pixel, sample, block, slot, seed=SEED, view=0):
+    out = O.philox([pixel, sample, block, 0], [seed, view])
+    return np.float32((out[slot] >> 8) * 2.0 ** -24)
+
+
+def _mirror_emitter_desc(bsdf=1, ks=(0.8, 0.5, 0.3), kd=(0.6, 0.4, 0.2), dcoef=0.5, emit=5.0, w=16, h=16):
+    """A mirror floor (y = 0, normal +y) seen from (0, 1, -1) at 45 degrees and a
+    2 x 2 emitter at y = 2 (normal -y) that every reflected camera ray meets at
+    z ~ 2. The camera's field of view is 0.5 degrees, so every pixel sees the
+    same mirror-then-emitter path."""
+    import cudart_amd as ca
+    cam = ca.camera_look_at((0, 1, -1), (0, 0, 0), (0, 1, 0), 0.5, w, h)
+    fp, fn = quad([-1, 0, -1], [-1, 0, 1], [1, 0, 1], [1, 0, -1])
+    lp, ln = quad([-1, 2, 1], [1, 2, 1], [1, 2, 3], [-1, 2, 3])
+    mats = [material(bsdf, kd=kd, ks=ks, dcoef=dcoef), material(5, kd=(1, 1, 1), ka=(emit, emit, emit))]
+    return OwnedDesc(np.concatenate([fp, lp]), np.concatenate([fn, ln]), [0, 0, 1, 1], mats, cam), cam
+
+
+def _render1(od, cam, w, h, c):
+    s = O.OracleScene(od.desc)
+    acc, cnt = s.render(cam, w, h, 0, 1, c)
+    s.close()
+    return acc.reshape(h * w, 3).astype(np.float64), cnt
+
+
+def test_mirror_then_emitter_is_ks_times_3ka():
+    # Bounce 0 on the mirror adds its (zero) emission and sets mask = ks / pdf(1)
+    # (:239-243); bounce 1 follows a SPECULAR vertex, so it adds mask * ka / A_L =
+    # ks * 3 Ka (ka = 3 Ka A_L, :158-161). At depth 3 the emitter's stale direction
+    # (EMISSIVE has no branch, Q4) leaves upward and adds mask * sky.
+    ks, ka = np.array([0.8, 0.5, 0.3]), 5.0
+    od, cam = _mirror_emitter_desc(ks=tuple(ks), emit=ka)
+    img, cnt = _render1(od, cam, 16, 16, O.cfg(max_depth=2))
+    np.testing.assert_allclose(img, np.broadcast_to(ks * 3 * ka, img.shape), rtol=2e-6)
+    assert cnt[1] == 256 and cnt[2] == 0  # one extension ray per path, no shadow rays
+    img, cnt = _render1(od, cam, 16, 16, O.cfg(max_depth=3))
+    np.testing.assert_allclose(img, np.broadcast_to(ks * (3 * ka + SKY), img.shape), rtol=2e-6)
+    # a diffuse floor in the same place adds no emission at bounce 1 (previous BSDF diffuse)
+    od, cam = _mirror_emitter_desc(bsdf=0, kd=(0, 0, 0))
+    img, _ = _render1(od, cam, 16, 16, O.cfg(max_depth=2))
+    assert (img == 0).all()
+
+
+def test_refractive_normal_incidence_takes_one_of_two_closed_forms():
+    # A refractive quad (IOR 1.5) at z = 0 facing the camera at z = 5, an emitter
+    # behind it at z = -2. At normal incidence Re = 24.25, Tr = -23.25, P = 12.375,
+    # RP = Re / P = 1.95960, TP = Tr / (1 - P) = 2.04396 (:464-493, :190-205).
+    # The lobe draw u = slot 0 of block 1: u > 0.25 transmits straight through,
+    # mask = ks * TP, and the emitter adds ks * TP * 3 Ka (the previous BSDF is
+    # REFRACTIVE); otherwise mask = RP, the vertex turns SPECULAR (mask *= ks) and
+    # the mirrored ray (+z) leaves to the sky: RP * ks * sky.
+    import cudart_amd as ca
+    W = H = 64
+    cam = ca.camera_look_at((0, 0, 5), (0, 0, 0), (0, 1, 0), 0.01, W, H)
+    gp, gn = quad([-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0])
+    lp, ln = quad([-1, -1, -2], [1, -1, -2], [1, 1, -2], [-1, 1, -2])
+    assert gn[0, 0, 2] == 1 and ln[0, 0, 2] == 1
+    ks, ka = np.array([0.9, 0.7, 0.5]), 2.0
+    mats = [material(4, ks=tuple(ks), ni=1.5), material(5, ka=(ka, ka, ka))]
+    od = OwnedDesc(np.concatenate([gp, lp]), np.concatenate([gn, ln]), [0, 0, 1, 1], mats, cam)
+    img, cnt = _render1(od, cam, W, H, O.cfg(max_depth=2))
+    Re, Tr = 24.25, -23.25
+    P = 0.25 + 0.5 * Re
+    RP, TP = Re / P, Tr / (1 - P)
+    transmit = ks * TP * 3 * ka
+    reflect = RP * ks * SKY
+    u = np.array([_draw(p, 0, 1, 0) for p in range(W * H)])
+    through = u > np.float32(0.25)
+    np.testing.assert_allclose(img[through], np.broadcast_to(transmit, img[through].shape), rtol=1e-5)
+    np.testing.assert_allclose(img[~through], np.broadcast_to(reflect, img[~through].shape), rtol=1e-5)
+    # the transmitted share is 0.75 (not P, not 0.25), within 5 binomial deviations
+    frac = through.mean()
+    assert abs(frac - 0.75) < 5 * math.sqrt(0.75 * 0.25 / (W * H)), frac
+    assert cnt[1] == W * H
+
+
+def test_diffspec_is_the_weighted_mixture_of_its_lobes():
+    # DIFFSPEC (:171-181): u = slot 0 of block 1; u < p takes the diffuse lobe with
+    # mask *= 1/p, else the specular lobe with mask *= 1/(1-p). Every later draw
+    # uses the same slots in the pure DIFFUSE and pure SPECULAR renders of the same
+    # material, so each DIFFSPEC pixel equals the DIFFUSE pixel / p or the SPECULAR
+    # pixel / (1 - p), as its own lobe draw says.
+    W = H = 32
+    p = 0.3
+    kw = dict(ks=(0.8, 0.5, 0.3), kd=(0.6, 0.4, 0.2), dcoef=p, w=W, h=H)
+    c = O.cfg(max_depth=3)
+    ds, _ = _render1(*_mirror_emitter_desc(bsdf=2, **kw), W, H, c)
+    dd, _ = _render1(*_mirror_emitter_desc(bsdf=0, **kw), W, H, c)
+    dspec, _ = _render1(*_mirror_emitter_desc(bsdf=1, **kw), W, H, c)
+    u = np.array([_draw(q, 0, 1, 0) for q in range(W * H)])
+    diffuse = u < np.float32(p)
+    np.testing.assert_allclose(ds[diffuse], dd[diffuse] / p, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(ds[~diffuse], dspec[~diffuse] / (1 - p), rtol=1e-5)
+    assert (dd[diffuse] > 0).any() and not np.allclose(dd[diffuse], dspec[diffuse])
+    frac = diffuse.mean()
+    assert abs(frac - p) < 5 * math.sqrt(p * (1 - p) / (W * H)), frac
+
+
+def test_roulette_kills_by_its_own_draw_and_rescales_survivors():
+    # Roulette from bounce 0 (rr_start = 0) on the mirror-then-emitter scene: after
+    # the mirror mask = ks, p = max(ks) = 0.8; the path ends if the roulette draw
+    # (slot 2 of block 2) is above p (:253-259), else mask = ks / p and the emitter
+    # adds (ks / p) * 3 Ka. The mean over pixels is ks * 3 Ka (unbiased).
+    W = H = 32
+    ks, ka = np.array([0.8, 0.5, 0.3]), 5.0
+    od, cam = _mirror_emitter_desc(ks=tuple(ks), emit=ka, w=W, h=H)
+    img, cnt = _render1(od, cam, W, H, O.cfg(max_depth=2, rr_start=0))
+    r = np.array([_draw(q, 0, 2, 2) for q in range(W * H)])
+    p = np.float32(0.8)
+    alive = ~(r > p)
+    np.testing.assert_allclose(img[alive], np.broadcast_to(ks / 0.8 * 3 * ka, img[alive].shape), rtol=2e-6)
+    assert (img[~alive] == 0).all()
+    assert cnt[1] == alive.sum()  # killed paths trace no extension ray
+    frac = alive.mean()
+    assert abs(frac - 0.8) < 5 * math.sqrt(0.8 * 0.2 / (W * H)), frac
+    mean = img.mean(0)
+    np.testing.assert_allclose(mean, ks * 3 * ka, rtol=5 * math.sqrt(0.2 / 0.8 / (W * H)))
+
+
+def test_roulette_starts_at_bounce_three_between_two_mirrors():
+    # A camera between two parallel mirrors (y = 0 facing up, y = 2 facing down)
+    # looking straight down: every path bounces until the depth limit or roulette.
+    # With ks = (0.8, 0.5, 0.3) the mask after bounce b is ks^(b+1) rescaled by the
+    # earlier roulette steps, so p = 0.8^4 at bounce 3 and 0.8 after. The extension
+    # ray count must equal the count predicted from each path's own roulette draws
+    # with roulette at bounces >= 3 (the reference's constant).
+    import cudart_amd as ca
+    W = H = 16
+    D = 8
+    cam = ca.camera_look_at((0, 1, 0), (0, 0, 0), (0, 0, 1), 0.01, W, H)
+    fp, fn = quad([-5, 0, -5], [-5, 0, 5], [5, 0, 5], [5, 0, -5])
+    cp, cn = quad([-5, 2, -5], [5, 2, -5], [5, 2, 5], [-5, 2, 5])
+    lp, ln = quad([49, 0, -1], [49, 0, 1], [51, 0, 1], [51, 0, -1])  # an emitter no path meets
+    assert fn[0, 0, 1] == 1 and cn[0, 0, 1] == -1
+    ks = np.array([0.8, 0.5, 0.3], np.float32)
+    mats = [material(1, ks=tuple(ks)), material(5, ka=(1, 1, 1))]
+    od = OwnedDesc(np.concatenate([fp, cp, lp]), np.concatenate([fn, cn, ln]), [0, 0, 0, 0, 1, 1], mats, cam)
+    img, cnt = _render1(od, cam, W, H, O.cfg(max_depth=D))
+    assert (img == 0).all()
+    want = 0
+    for q in range(W * H):
+        mask = np.ones(3, np.float32)
+        for b in range(D):
+            if b > 0:
+                want += 1  # bounce b traces an extension ray
+            mask = mask * ks
+            if b >= 3:
+                pb = mask.max()
+                if _draw(q, 0, 2 + 2 * b, 2) > pb:
+                    break
+                mask = mask / pb
+    assert cnt[1] == want, (int(cnt[1]), want)
+    assert want < W * H * (D - 1)  # some paths ended by roulette
