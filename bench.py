@@ -16,15 +16,22 @@ image must equal the wavefront's bit for bit (so the RMSE against any reference
 is the same), and the ratio of the two rates is reported.
 
 Roofline of the dominant kernel (wf_extend, closest-hit traversal of every primary
-and extension ray): algorithmic bytes per launch
-    112 B x four-wide nodes + 48 B x triangle records + 64 B x shading records
-of the closest-hit rays (device counters of the instrumented frame, shadow-ray
-share removed), divided by the average extend launch time measured with HIP events
-around every extend launch on the renderer's stream (a frame rendered with the
-chunk's kernels launched directly instead of through the graph); peak = 8 TB/s.
-`traffic` = measured HBM bytes per extend launch from rocprofv3 FETCH_SIZE and
-WRITE_SIZE passes scaled by the same counters on a plain copy kernel (read from
-the committed summary under profiles/, null when absent).
+and extension ray). The traversal is irregular pointer chasing with no dense
+contraction, so the roofline is HBM's (peak 8 TB/s). `achieved` = measured HBM
+bytes per extend launch (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled
+by the same counter on a plain 1 GiB copy; read from the committed summary under
+profiles/) divided by the average extend launch time measured live with HIP
+events around every extend launch on the renderer's stream (a frame rendered with
+the chunk's kernels launched directly instead of through the graph); `frac` =
+achieved / peak. Beside it: the kernel's algorithmic bytes per launch
+    64 B x four-wide node visits + 48 B x triangle records tested
+    + 48 B x hits (the winning record re-read for u, v, material)
+    + 52 B x rays (32 B queue entry read, 20 B hit record written)
+(device counters of the instrumented frame, shadow rays removed), most of which
+the L2 and the MALL serve; and the instruction-issue fraction: the kernel's wave64
+VALU instructions per launch (committed SQ_INSTS_VALU summary) per second of the
+live launch time, over the rate a calibration kernel of independent fp32 FMAs
+reaches on the same GPU (bench.py --issue-probe, committed summary).
 
 Adjoint leg: path-replay gradients of the L2 image loss w.r.t. 16 materials'
 albedo and emission on the 100k-triangle Cornell-like scene at 512x512, 64
@@ -32,8 +39,13 @@ samples replayed (the forward render keeps each sample's radiance): grad-samples
 = pixels x samples / adjoint device time.
 
 CPU baseline: the oracle (C++ restatement of the reference --cpu renderer, fp32,
-OpenMP) on a bounded sample of pixels of the same frame, in Mrays/s with the
-thread count.
+OpenMP over every core this process may run on, as the reference's
+`omp parallel for` uses all threads) on a bounded sample of pixels of the same
+frame, in Mrays/s, with the thread count, the host's CPU count and model.
+
+Image error: RMSE of the timed frame against the oracle's fp64 8,192-sample
+estimate of 4,096 fixed pixels (tests/golden/spheres1m_1080p_fp64_reference.npz,
+made by tests/golden/make_rmse_reference.py), with that estimate's own noise.
 """
 from __future__ import annotations
 
@@ -53,6 +65,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mrays/s + ms/frame @1080p 128spp 1M tris; adjoint grad-samples/s; CPU-relative"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "wf_extend_spheres1m_1080p_hbm_traffic.txt"
+ISSUE_FILE = ROOT / "profiles" / "wf_extend_wf_shadow_spheres1m_1080p_sq_counters.txt"
+RMSE_FILE = ROOT / "tests" / "golden" / "spheres1m_1080p_fp64_reference.npz"
 
 
 def parse():
@@ -82,6 +96,8 @@ def parse():
                    help="with --frames-only: launch the wavefront chunks' kernels directly, not through graphs")
     p.add_argument("--copy-probe", action="store_true",
                    help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
+    p.add_argument("--issue-probe", action="store_true",
+                   help="run the fp32-FMA issue calibration kernel 5 times and exit (counter calibration)")
     a = p.parse_args()
     size = {"spheres1m": (1920, 1080, 128, 8), "large": (3840, 2160, 512, 8), "cornell": (256, 256, 16, 5)}
     w, h, spp, depth = size[a.workload]
@@ -102,6 +118,45 @@ def copy_probe():
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
+
+
+def issue_probe():
+    """The VALU issue rate of independent fp32 FMAs (8 chains per thread, full
+    occupancy, no memory traffic): 16,384 workgroups x 4 waves x 4,096 steps x 8 FMAs."""
+    import ctypes as C
+
+    # HIP runtime of the in-tree libraries (not torch's), so no torch in this process.
+    lib = C.CDLL(str(ROOT / "cudart_amd" / "lib" / "libprt_probe.so"))
+    lib.prt_probe_valu.restype = C.c_int
+    lib.prt_probe_valu.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]
+    blocks, iters, launches = 16384, 4096, 5
+    ms = C.c_float()
+    rc = lib.prt_probe_valu(blocks, iters, 1, C.byref(ms))  # warm-up
+    rc = rc or lib.prt_probe_valu(blocks, iters, launches, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"issue probe failed: hipError {-rc}")
+    fma = blocks * 4 * iters * 8
+    print(json.dumps({"issue_probe_fma_instructions_per_launch": fma, "launches": launches,
+                      "ms_per_launch": round(ms.value / launches, 4),
+                      "fma_instructions_per_s": fma / (ms.value / launches * 1e-3)}))
+
+
+def host_cores() -> int:
+    """Cores this process may run on (the box's CPU share), as OpenMP's default uses."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(wl, target_s: float, threads: int):
@@ -130,15 +185,37 @@ def cpu_baseline(wl, target_s: float, threads: int):
     return {"value": round(rays / dt / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{n} random pixels x samples 0..{spp - 1} of the same {wl.width}x{wl.height} frame: "
                       f"{rays} rays in {dt:.1f} s on {threads} OpenMP threads",
-            "ms_per_frame_extrapolated": round(frame_ms, 1)}
+            "ms_per_frame_extrapolated": round(frame_ms, 1),
+            "host_cpus": os.cpu_count(), "affinity_cpus": host_cores(), "cpu_model": cpu_model()}
+
+
+def summary_value(path: Path, key: str):
+    """`key = value` from a committed rocprofv3 summary under profiles/, or None."""
+    if not path.exists():
+        return None
+    m = re.search(rf"^{re.escape(key)}\s*=\s*([0-9.eE+-]+)", path.read_text(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def measured_traffic():
     """HBM bytes per wf_extend launch from the committed rocprofv3 summary, or None."""
-    if not TRAFFIC_FILE.exists():
+    return summary_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
+
+
+def rmse_against_reference(image, wl):
+    """RMSE of a width x height x 3 image against the committed fp64 reference."""
+    if not RMSE_FILE.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text(), re.M)
-    return float(m.group(1)) if m else None
+    ref = np.load(RMSE_FILE, allow_pickle=False)
+    if (int(ref["width"]), int(ref["height"]), int(ref["max_depth"])) != (wl.width, wl.height, wl.max_depth):
+        return None
+    px = ref["pixels"]
+    got = image.reshape(-1, 3)[px].astype(np.float64)
+    err = got - ref["rgb"]
+    return {"value": round(float(np.sqrt(np.mean(err ** 2))), 6), "pixels": int(len(px)),
+            "reference_spp": int(ref["spp"]), "reference_noise_rms": round(float(np.sqrt(np.mean(ref["stderr"] ** 2))), 6),
+            "mean_radiance": round(float(ref["rgb"].mean()), 6), "mean_error": float(err.mean()),
+            "reference": "oracle fp64, tests/golden/spheres1m_1080p_fp64_reference.npz"}
 
 
 def timed_frames(r, spp, warmup, steps):
@@ -218,6 +295,9 @@ def main():
     if a.copy_probe:
         copy_probe()
         return
+    if a.issue_probe:
+        issue_probe()
+        return
     import cudart_amd as ca
     from cudart_amd import workloads as W
 
@@ -269,15 +349,16 @@ def main():
         rp.close()
         launches = sp.kernel_launches[0]
         launch_ms = sp.kernel_ms[0] / launches
+        ext_rays = st.primary_rays + st.extension_rays
         ext_bytes = (nb * (st.nodes_visited - st.anyhit_nodes) + 48 * (st.tris_tested - st.anyhit_tris)
-                     + 64 * st.hits)
+                     + 48 * st.hits + 52 * ext_rays)
         bytes_per_launch = ext_bytes / launches
         kernel = "wf_extend"
         kernel_times = {"extend": round(sp.kernel_ms[0], 2), "shade": round(sp.kernel_ms[1], 2),
                         "shadow": round(sp.kernel_ms[2], 2), "launches_each": launches,
                         "profiled_frame_ms": round(sp.last_render_ms, 2)}
-        per_ray = {"nodes": round((st.nodes_visited - st.anyhit_nodes) / (st.primary_rays + st.extension_rays), 2),
-                   "tris": round((st.tris_tested - st.anyhit_tris) / (st.primary_rays + st.extension_rays), 2)}
+        per_ray = {"nodes": round((st.nodes_visited - st.anyhit_nodes) / ext_rays, 2),
+                   "tris": round((st.tris_tested - st.anyhit_tris) / ext_rays, 2)}
     else:
         launches = (a.spp + per - 1) // per
         launch_ms = dev_ms / launches
@@ -285,8 +366,35 @@ def main():
         kernel = "megakernel"
         kernel_times = None
         per_ray = {"nodes": round(st.nodes_visited / rays, 2), "tris": round(st.tris_tested / rays, 2)}
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic() if wave and a.workload == "spheres1m" else None  # summary is of that frame
+    algorithmic = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    # The counter summaries are of the default frame (1M scene, 1080p, 128 spp).
+    default_frame = (wave and a.workload == "spheres1m" and (a.width, a.height, a.spp, a.depth, a.instances)
+                     == (1920, 1080, 128, 8, 200))
+    traffic = measured_traffic() if default_frame else None
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic, "kernel": kernel,
+                "launch_ms": round(launch_ms, 4),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "algorithmic_GBs": round(algorithmic, 1), "per_ray": per_ray, "kernel_ms_per_frame": kernel_times}
+    if traffic is not None:
+        achieved = traffic / (launch_ms * 1e-3) / 1e9
+        roofline.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4),
+                        basis="measured HBM bytes per launch (profiles/) / live launch time",
+                        traffic_over_algorithmic=round(traffic / bytes_per_launch, 4))
+        valu = summary_value(ISSUE_FILE, "wf_extend_valu_instructions_per_launch")
+        peak = summary_value(ISSUE_FILE, "issue_probe_valu_instructions_per_s")
+        lanes = summary_value(ISSUE_FILE, "wf_extend_active_lanes_per_valu")
+        if valu and peak:
+            rate = valu / (launch_ms * 1e-3)
+            roofline["issue"] = {"achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+                                 "unit": "G wave64 VALU instructions/s", "frac": round(rate / peak, 4),
+                                 "active_lanes_per_valu": lanes,
+                                 "peak_source": "calibration kernel of independent fp32 FMAs on the same GPU "
+                                                "(bench.py --issue-probe; profiles/)"}
+    else:
+        # No counter summary for this configuration: the algorithmic rate stands in.
+        roofline.update(achieved=round(algorithmic, 1), frac=round(algorithmic / HBM_PEAK_GBS, 4),
+                        basis="algorithmic bytes per launch / live launch time (no counter summary for this "
+                              "configuration; caches serve much of it)")
 
     out = {
         "metric": METRIC,
@@ -308,10 +416,7 @@ def main():
         "rays_per_frame": int(rays),
         "rays_by_kind": {"primary": int(st.primary_rays), "extension": int(st.extension_rays),
                          "shadow": int(st.shadow_rays)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
-                     "launch_ms": round(launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
-                     "per_ray": per_ray, "kernel_ms_per_frame": kernel_times},
+        "roofline": roofline,
         "bvh": {"nodes": info.num_nodes, "node_read_bytes": nb, "max_depth": info.max_depth,
                 "sah_cost": round(info.sah_cost, 2), "build_s": round(build_s, 2)},
     }
@@ -326,10 +431,15 @@ def main():
                              "spp_per_launch": per, "steps": m_steps,
                              "image_equal_to_wavefront": bool(np.array_equal(image, m_image))}
         out["wavefront_over_megakernel"] = round(value / m_value, 3)
+    if a.workload == "spheres1m":
+        rm_ = rmse_against_reference(image, wl)
+        if rm_ is not None:
+            out["rmse_vs_fp64_oracle"] = rm_
+    threads = a.threads or host_cores()
     if not a.no_adjoint:
-        out["adjoint"] = adjoint_leg(0 if a.no_cpu else (a.threads or min(16, os.cpu_count() or 1)))
+        out["adjoint"] = adjoint_leg(0 if a.no_cpu else threads)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds, a.threads or min(16, os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds, threads)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out))
 

@@ -15,13 +15,13 @@ import numpy as np
 from . import _capi
 from ._capi import (DIFFSPEC, DIFFUSE, EMISSIVE, INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, MICROFACET,
                     NO_TEXTURE, REFRACTIVE, SPECULAR, Camera, Material, PrtError, RenderConfig, SceneDesc, Stats,
-                    check, lib)
+                    Tune, check, lib)
 
 __all__ = [
     "Mesh", "Scene", "ParallelRenderer", "Camera", "Material", "RenderConfig", "PrtError", "camera_look_at",
     "camera_from_json", "default_config", "device_count", "scene_arrays", "DIFFUSE", "SPECULAR", "DIFFSPEC",
     "MICROFACET", "REFRACTIVE", "EMISSIVE", "INTEGRATOR_MEGAKERNEL", "INTEGRATOR_WAVEFRONT", "NO_TEXTURE",
-    "write_png", "write_pfm",
+    "write_png", "write_pfm", "Tune",
 ]
 
 
@@ -158,6 +158,7 @@ class Scene:
         self.num_triangles = desc.num_triangles
         self.num_materials = desc.num_materials
         self.camera = Camera.from_buffer_copy(desc.camera)
+        self._params = {}  # material index -> (kd, emission) of its last set_material_params
 
     def bvh_info(self) -> _capi.BvhInfo:
         i = _capi.BvhInfo()
@@ -179,8 +180,26 @@ class Scene:
         return n.value, a.value
 
     def set_material_params(self, index: int, kd, emission: float) -> None:
-        k = np.asarray(kd, np.float32)
-        check(lib().prt_scene_set_material_params(self._h, index, _capi.fptr(k), float(emission)))
+        k = np.ascontiguousarray(kd, np.float32).reshape(-1)
+        if k.size != 3:
+            raise ValueError(f"kd must hold 3 values (RGB), got {k.size}")
+        if not 0 <= int(index) < self.num_materials:
+            raise ValueError(f"material index {index} out of range 0..{self.num_materials - 1}")
+        check(lib().prt_scene_set_material_params(self._h, int(index), _capi.fptr(k), float(emission)))
+        self._params[int(index)] = (k.copy(), np.float32(emission))
+
+    def has_material_params(self, kd, emission) -> bool:
+        """Whether the last set_material_params of every material set exactly these
+        kd (M, 3) and emission (M,) values."""
+        kd = np.asarray(kd, np.float32).reshape(-1, 3)
+        em = np.asarray(emission, np.float32).reshape(-1)
+        if len(kd) != self.num_materials or len(em) != self.num_materials:
+            return False
+        for m in range(self.num_materials):
+            cur = self._params.get(m)
+            if cur is None or not np.array_equal(cur[0], kd[m]) or cur[1] != em[m]:
+                return False
+        return True
 
     def close(self) -> None:
         if self._h:
@@ -263,14 +282,22 @@ class ParallelRenderer:
     def reset_stats(self) -> None:
         check(lib().prt_reset_stats(self._h))
 
+    def _image_arg(self, a, name: str) -> np.ndarray:
+        """a as a contiguous float32 (height, width, 3) array; anything else is refused
+        (the C side reads width * height * 3 floats)."""
+        arr = np.ascontiguousarray(a, np.float32)
+        if arr.shape != (self.height, self.width, 3):
+            raise ValueError(f"{name} must have shape ({self.height}, {self.width}, 3), got {arr.shape}")
+        return arr
+
     def adjoint(self, target: np.ndarray, spp: int, image: np.ndarray | None = None):
         """Path-replay gradients of mean((I - target)^2) w.r.t. kd (M,3) and emission (M,)."""
         M = self.scene.num_materials
         d_alb = np.zeros((M, 3), np.float32)
         d_em = np.zeros(M, np.float32)
         loss = C.c_float()
-        tgt = np.ascontiguousarray(target, np.float32)
-        img = None if image is None else np.ascontiguousarray(image, np.float32)
+        tgt = self._image_arg(target, "target")
+        img = None if image is None else self._image_arg(image, "image")
         check(lib().prt_adjoint(self._h, _capi.fptr(tgt), None if img is None else _capi.fptr(img), spp,
                                 _capi.fptr(d_alb), _capi.fptr(d_em), C.byref(loss)))
         return d_alb, d_em, loss.value
@@ -281,7 +308,7 @@ class ParallelRenderer:
         M = self.scene.num_materials
         d_alb = np.zeros((M, 3), np.float32)
         d_em = np.zeros(M, np.float32)
-        g = np.ascontiguousarray(grad_image, np.float32)
+        g = self._image_arg(grad_image, "grad_image")
         check(lib().prt_adjoint_grad(self._h, _capi.fptr(g), float(hess), spp, _capi.fptr(d_alb),
                                      _capi.fptr(d_em)))
         return d_alb, d_em
@@ -327,7 +354,8 @@ def render_batch(jobs: list, config: RenderConfig | None = None, tune: "_capi.Tu
 
 
 def autotune(jobs: list, config: RenderConfig | None = None, **cfg):
-    """Times queue sizes and persistent occupancy on a 2-spp trial of `jobs`; returns (Tune, ms)."""
+    """Searches traversal occupancy, camera-path grouping and chunk count on an 8-row
+    trial of `jobs` (prt_autotune); returns (Tune, trial ms)."""
     c = config if config is not None else default_config(**cfg)
     arr = (_capi.Job * len(jobs))()
     for i, j in enumerate(jobs):

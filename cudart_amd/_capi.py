@@ -49,7 +49,7 @@ class SceneDesc(C.Structure):
 class RenderConfig(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32), ("view_id", C.c_uint32),
                 ("device", C.c_int32), ("integrator", C.c_uint32), ("spp_per_launch", C.c_uint32),
-                ("count_rays", C.c_uint32), ("wavefront_chunk", C.c_uint32), ("wavefront_queue", C.c_uint32),
+                ("count_rays", C.c_uint32), ("wavefront_group", C.c_uint32), ("wavefront_queue", C.c_uint32),
                 ("profile_kernels", C.c_uint32), ("wavefront_blocks_per_cu", C.c_uint32),
                 ("keep_samples", C.c_uint32)]
 
@@ -73,7 +73,7 @@ class Job(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("queue_slots", C.c_uint32), ("chunk_spp", C.c_uint32), ("block_size", C.c_uint32),
+    _fields_ = [("queue_slots", C.c_uint32), ("group_side", C.c_uint32), ("block_size", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32)]
 
 

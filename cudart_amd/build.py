@@ -82,6 +82,16 @@ def build_library(jobs: int | None = None) -> Path:
     return lib
 
 
+def build_probe() -> Path:
+    """libprt_probe.so: the instruction-issue calibration kernel of bench.py --issue-probe."""
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    lib = LIB_DIR / "libprt_probe.so"
+    tmp = LIB_DIR / "libprt_probe.so.tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "probe" / "issue_probe.hip"), "-o", str(tmp)])
+    os.replace(tmp, lib)
+    return lib
+
+
 def build_cli() -> Path:
     """The product CLI (`-f -s -w -h`), linked against libprt.so."""
     BIN_DIR.mkdir(parents=True, exist_ok=True)
@@ -111,11 +121,13 @@ def build_oracle_cli() -> Path:
 
 
 def build_all() -> None:
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
         f_lib = ex.submit(build_library)
         f_orc = ex.submit(build_oracle)
+        f_probe = ex.submit(build_probe)
         f_lib.result()
         f_orc.result()
+        f_probe.result()
     build_cli()
     build_oracle_cli()
 

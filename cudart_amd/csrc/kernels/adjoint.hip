@@ -424,6 +424,36 @@ bool primal_in_replay() {
     const char* v = std::getenv("PRT_ADJOINT_PRIMAL_IN_REPLAY");
     return v && v[0] == '1';
 }
+
+// The replay's arguments for the renderer's view: seed, view, depth and the scene
+// as the renderer holds them; grad = W*H*3 dLoss/dI on the device.
+AdjArgs adjoint_args(const prt_renderer* r, const float* grad, float hess, uint32_t spp, double* partial) {
+    AdjArgs a;
+    a.sc = r->sc;
+    a.cam = cam_args(r->cam);
+    a.path = PathArgs{r->cfg.seed, r->cfg.view_id, r->cfg.max_depth, r->cfg.rr_start};
+    a.width = r->w;
+    a.height = r->h;
+    a.spp = spp;
+    a.tiles_x = r->tiles_x;
+    a.tiles = r->tiles;
+    a.grad = grad;
+    // The forward pass's per-sample radiance, when it was kept for exactly these
+    // samples with the scene's current materials (same seed, view and depth: the
+    // renderer's configuration).
+    a.samples = r->samples && r->samples_ok && r->samples_done >= spp && spp <= r->spp &&
+                        r->samples_version == r->scene->mats_version
+                    ? r->samples
+                    : nullptr;
+    a.norm = 1.0 / double(spp);
+    a.half_hess = double(hess) / (2.0 * double(spp) * double(spp));
+    a.num_params = 4 * r->scene->num_materials;
+    a.num_materials = r->scene->num_materials;
+    a.partial = partial;
+    a.spill = r->spill;
+    a.spill_stride = r->spill_threads;
+    return a;
+}
 }  // namespace
 
 }  // namespace prt
@@ -457,29 +487,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_part), size_t(grid) * P * sizeof(double)));
     ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_out), P * sizeof(double)));
     ADJ_TRY(hipMemcpy(d_grad, grad_rgb, px * 3 * sizeof(float), hipMemcpyHostToDevice));
-    a.sc = r->sc;
-    a.cam = cam_args(r->cam);
-    a.path = PathArgs{r->cfg.seed, r->cfg.view_id, r->cfg.max_depth, r->cfg.rr_start};
-    a.width = r->w;
-    a.height = r->h;
-    a.spp = spp;
-    a.tiles_x = r->tiles_x;
-    a.tiles = r->tiles;
-    a.grad = d_grad;
-    // The forward pass's per-sample radiance, when it was kept for exactly these
-    // samples with the scene's current materials (same seed, view and depth: the
-    // renderer's configuration).
-    a.samples = r->samples && r->samples_ok && r->samples_done >= spp && spp <= r->spp &&
-                        r->samples_version == r->scene->mats_version
-                    ? r->samples
-                    : nullptr;
-    a.norm = 1.0 / double(spp);
-    a.half_hess = double(hess) / (2.0 * double(spp) * double(spp));
-    a.num_params = P;
-    a.num_materials = M;
-    a.partial = d_part;
-    a.spill = r->spill;
-    a.spill_stride = r->spill_threads;
+    a = adjoint_args(r, d_grad, hess, spp, d_part);
     // Without kept radiance, a wavefront forward pass first writes each sample's
     // radiance to a buffer of this call (when that fits in a quarter of the free
     // memory), and the replay kernel only replays; otherwise each lane traces its

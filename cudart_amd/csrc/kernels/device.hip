@@ -176,6 +176,8 @@ extern "C" int prt_renderer_create(prt_scene* scene, const prt_camera* cam, uint
     clear_error();
     if (!scene || !out) return fail(PRT_ERR_INVALID_ARG, "null argument");
     if (w == 0 || h == 0 || w > 65536 || h > 65536) return fail(PRT_ERR_INVALID_ARG, "image size out of range");
+    // Pixel indices and counts are 32-bit throughout (kernels, tonemap, planner).
+    if (uint64_t(w) * h >= (uint64_t(1) << 32)) return fail(PRT_ERR_LIMIT, "image of 2^32 pixels or more");
     if (scene->lights.empty())
         return fail(PRT_ERR_NO_EMITTERS, "scene has no EMISSIVE triangles (the reference divides by a zero light area)");
     prt_render_config c;

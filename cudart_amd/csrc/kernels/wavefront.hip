@@ -128,6 +128,7 @@ struct WfArgs {
     int* spill;
     uint32_t spill_stride;
     uint32_t max_depth;
+    uint32_t group_lg;  // log2 of the forced camera-path block side, or kNone: choose per tile
 };
 
 struct WavefrontState {
@@ -275,6 +276,9 @@ static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase th
 #ifndef PRT_EXTEND_WAVES
 #define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(8)))
 #endif
+#ifndef PRT_SHADE_WAVES
+#define PRT_SHADE_WAVES
+#endif
 
 // Adds the wave's counts to the counter slots; ANYHIT also books nodes and
 // triangles as the shadow (any-hit) kernel's share.
@@ -334,14 +338,19 @@ __global__ void wf_advance(WfArgs a) {
 struct PathGroup {
     uint32_t lg;  // log2 of the block side g; kNone: row order
 };
-__device__ __forceinline__ PathGroup tile_group(const WfTile& t, uint32_t width) {
+// force_lg (not kNone) asks for g = 2^force_lg; a tile it does not divide takes the
+// smallest g that does.
+__device__ __forceinline__ PathGroup tile_group(const WfTile& t, uint32_t width, uint32_t force_lg) {
     const uint32_t w = t.p1 - t.p0, ns = t.s1 - t.s0;
     if (t.p0 % width != 0u || w % width != 0u) return PathGroup{kNone};
     const uint32_t rows = w / width;
-    for (uint32_t lg = 0; lg <= 3; ++lg) {
+    auto fits = [&](uint32_t lg) {
         const uint32_t g = 1u << lg, gs = 64u >> (2u * lg);
-        if (ns % gs == 0u && width % g == 0u && rows % g == 0u) return PathGroup{lg};
-    }
+        return ns % gs == 0u && width % g == 0u && rows % g == 0u;
+    };
+    if (force_lg <= 3u && fits(force_lg)) return PathGroup{force_lg};
+    for (uint32_t lg = 0; lg <= 3; ++lg)
+        if (fits(lg)) return PathGroup{lg};
     return PathGroup{kNone};
 }
 // Position r of a grouped tile of w pixels -> (pixel offset from p0, sample offset from s0).
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(256) void wf_generate(WfArgs a) {
         const WfJob& j = a.jobs[t.job];
         const uint32_t w = t.p1 - t.p0;
         const uint32_t r = slot - t.slot0;
-        const PathGroup pg = tile_group(t, j.width);
+        const PathGroup pg = tile_group(t, j.width, a.group_lg);
         const uint2 ps = pg.lg != kNone ? group_to_linear(r, w, j.width, pg) : make_uint2(r % w, r / w);
         const uint32_t pixel = t.p0 + ps.x, sample = t.s0 + ps.y;
         const int x = int(pixel % j.width), y = int(pixel / j.width);
@@ -472,41 +481,67 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
 
+// Shadow-queue classes: a workgroup's shadow rays are appended grouped by the part
+// of the scene's light list their light sample lies in (class = position * K /
+// number of lights), so the 64-ray batches the occlusion kernel fetches mostly aim
+// at one light. With several emitters far apart (two 4 x 4 quads 10 units apart in
+// the 1M-triangle scene), the rays of one shading point otherwise split between
+// subtrees and the lanes of a wave part ways. The order of the shadow queue does not
+// change any result: each entry updates only its own path.
+#ifndef PRT_SHADOW_CLASSES
+#define PRT_SHADOW_CLASSES 4
+#endif
+constexpr uint32_t kShadowClasses = PRT_SHADOW_CLASSES;
+static_assert(kShadowClasses >= 1 && kShadowClasses <= 8, "shadow classes out of range");
+
 // Appends to two queues for a whole workgroup with one atomic per queue: lanes
-// with more / nee get consecutive positions. Returns (next-queue, shadow) positions.
-__device__ __forceinline__ uint2 block_append(bool more, bool nee, uint32_t* ctr_next, uint32_t* ctr_shadow) {
-    __shared__ uint32_t s_off[2][kShadeBlock / 64];
+// with more get consecutive next-queue positions, lanes with nee consecutive
+// shadow positions ordered by shadow class (then wave, then lane). Returns
+// (next-queue, shadow) positions.
+__device__ __forceinline__ uint2 block_append(bool more, bool nee, uint32_t cls, uint32_t* ctr_next,
+                                              uint32_t* ctr_shadow) {
+    constexpr uint32_t kWaves = kShadeBlock / 64;
+    __shared__ uint32_t s_off0[kWaves];
+    __shared__ uint32_t s_off1[kShadowClasses][kWaves];
     __shared__ uint32_t s_base[2];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    const unsigned long long m0 = __ballot(more), m1 = __ballot(nee);
-    if (lane == 0) {
-        s_off[0][wave] = uint32_t(__popcll(m0));
-        s_off[1][wave] = uint32_t(__popcll(m1));
+    const unsigned long long m0 = __ballot(more);
+    unsigned long long mine = 0ull;  // this lane's class mask
+#pragma unroll
+    for (uint32_t c = 0; c < kShadowClasses; ++c) {
+        const unsigned long long mc = __ballot(nee && cls == c);
+        if (cls == c) mine = mc;
+        if (lane == 0) s_off1[c][wave] = uint32_t(__popcll(mc));
     }
+    if (lane == 0) s_off0[wave] = uint32_t(__popcll(m0));
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t0 = 0, t1 = 0;
-        for (int w = 0; w < kShadeBlock / 64; ++w) {
-            uint32_t c0 = s_off[0][w], c1 = s_off[1][w];
-            s_off[0][w] = t0;
-            s_off[1][w] = t1;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t c0 = s_off0[w];
+            s_off0[w] = t0;
             t0 += c0;
-            t1 += c1;
         }
+        for (uint32_t c = 0; c < kShadowClasses; ++c)
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                const uint32_t c1 = s_off1[c][w];
+                s_off1[c][w] = t1;
+                t1 += c1;
+            }
         s_base[0] = t0 ? atomicAdd(ctr_next, t0) : 0u;
         s_base[1] = t1 ? atomicAdd(ctr_shadow, t1) : 0u;
     }
     __syncthreads();
     const unsigned long long lt = (1ull << lane) - 1ull;
     uint2 r;
-    r.x = s_base[0] + s_off[0][wave] + uint32_t(__popcll(m0 & lt));
-    r.y = s_base[1] + s_off[1][wave] + uint32_t(__popcll(m1 & lt));
+    r.x = s_base[0] + s_off0[wave] + uint32_t(__popcll(m0 & lt));
+    r.y = nee ? s_base[1] + s_off1[cls][wave] + uint32_t(__popcll(mine & lt)) : 0u;
     __syncthreads();  // s_off / s_base are reused by the next iteration
     return r;
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
+__global__ __launch_bounds__(kShadeBlock) PRT_SHADE_WAVES void wf_shade(WfArgs a, uint32_t it) {
     const uint32_t q = it & 1u;
     const uint32_t n = a.ctr[kExt0 + q];
     // Read once, before this kernel's queue atomics start on the counters' cache line.
@@ -561,7 +596,12 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade(WfArgs a, uint32_t it) {
             }
         });
         if (valid && !more) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
-        const uint2 pos = block_append(more, want_nee, &a.ctr[kExt0 + (q ^ 1u)], &a.ctr[kShadow]);
+        uint32_t cls = 0;
+        if (want_nee && kShadowClasses > 1) {
+            const uint32_t nl = a.jobs[pj >> 24].sc.num_lights;
+            cls = uint32_t((uint64_t(ns.pick) * kShadowClasses) / nl);
+        }
+        const uint2 pos = block_append(more, want_nee, cls, &a.ctr[kExt0 + (q ^ 1u)], &a.ctr[kShadow]);
         const uint32_t qi = pos.x;
         if (more) {
             a.qa[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, A.w);
@@ -893,10 +933,17 @@ int run_profiled(const WfArgs& a, hipStream_t stream, size_t nchunks, uint32_t m
 }
 
 // Renders every job of the plan on `stream`. Returns PRT_OK or an error.
+// group_side: camera-path block side g for wf_generate (1, 2, 4, 8; 0 = choose per tile).
 int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::vector<WfJob>& jobs,
                   const std::vector<PlanJob>& pjobs, uint32_t max_depth, uint32_t capacity, uint32_t blocks_per_cu,
-                  bool count, RunStats* rs, bool profile = false) {
+                  uint32_t group_side, bool count, RunStats* rs, bool profile = false) {
     if (max_depth == 0 || max_depth > uint32_t(kMaxIter)) return fail(PRT_ERR_INVALID_ARG, "max_depth out of range");
+    uint32_t group_lg = dev::kNone;
+    if (group_side != 0) {
+        if (group_side != 1 && group_side != 2 && group_side != 4 && group_side != 8)
+            return fail(PRT_ERR_INVALID_ARG, "wavefront group side must be 0, 1, 2, 4 or 8");
+        group_lg = group_side == 1 ? 0u : group_side == 2 ? 1u : group_side == 4 ? 2u : 3u;
+    }
     // Queue entries pack pixel | job << 24.
     if (jobs.size() > 256) return fail(PRT_ERR_LIMIT, "more than 256 jobs in one batch");
     for (const PlanJob& pj : pjobs)
@@ -930,6 +977,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     a.tiles = w.d_tiles;
     a.chunks = w.d_chunks;
     a.max_depth = max_depth;
+    a.group_lg = group_lg;
     a.spill_stride = uint32_t(pblocks) * kBlock;
 
     hipDeviceProp_t prop;
@@ -1011,7 +1059,8 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
     RunStats rs;
     int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap,
-                           r->cfg.wavefront_blocks_per_cu, r->cfg.count_rays != 0, &rs, r->cfg.profile_kernels != 0);
+                           r->cfg.wavefront_blocks_per_cu, r->cfg.wavefront_group, r->cfg.count_rays != 0, &rs,
+                           r->cfg.profile_kernels != 0);
     if (rc != PRT_OK) return rc;
     // Fold the wavefront's counters into the renderer's.
     unsigned long long host[kCntNum];
@@ -1046,7 +1095,7 @@ int wavefront_samples(prt_renderer* r, uint32_t spp, float4* accum, float4* samp
     const uint32_t cap = r->cfg.wavefront_queue ? r->cfg.wavefront_queue : default_capacity(total);
     RunStats rs;
     const int rc = run_wavefront(*r->wf, r->cfg.device, r->stream, jobs, pj, r->cfg.max_depth, cap,
-                                 r->cfg.wavefront_blocks_per_cu, false, &rs);
+                                 r->cfg.wavefront_blocks_per_cu, r->cfg.wavefront_group, false, &rs);
     // A megakernel renderer does not keep the queues of a one-off forward pass.
     if (!had_state && r->cfg.integrator != PRT_INTEGRATOR_WAVEFRONT) release_wavefront(r);
     return rc;
@@ -1077,6 +1126,9 @@ int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCt
         if (!jb.scene) return fail(PRT_ERR_INVALID_ARG, "job without a scene");
         if (jb.scene->lights.empty()) return fail(PRT_ERR_NO_EMITTERS, "job scene has no emitters");
         if (jb.camera.xpixels <= 0 || jb.camera.ypixels <= 0) return fail(PRT_ERR_INVALID_ARG, "job camera size");
+        if (jb.spp == 0) return fail(PRT_ERR_INVALID_ARG, "job with zero samples per pixel");
+        if (uint64_t(jb.camera.xpixels) * uint64_t(jb.camera.ypixels) > (1u << 24))
+            return fail(PRT_ERR_LIMIT, "a view with more than 2^24 pixels");
         WfJob w;
         int rc = device_scene(jb.scene, c.device, w.sc);
         if (rc != PRT_OK) return rc;
@@ -1114,6 +1166,8 @@ using namespace prt;
 extern "C" int prt_render_batch(prt_job* jobs, uint32_t n, const prt_render_config* cfg, const prt_tune* tune,
                                 prt_stats* stats) {
     clear_error();
+    if (tune && tune->block_size != 0 && tune->block_size != uint32_t(kBlock))
+        return fail(PRT_ERR_INVALID_ARG, "prt_tune.block_size: the traversal kernels are built for 64 threads");
     prt_render_config c;
     if (cfg) c = *cfg;
     else prt_render_config_default(&c);
@@ -1136,7 +1190,7 @@ extern "C" int prt_render_batch(prt_job* jobs, uint32_t n, const prt_render_conf
     const uint32_t cap = tune && tune->queue_slots ? tune->queue_slots : default_capacity(b.total);
     RunStats rs;
     rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, cap, tune ? tune->blocks_per_cu : 0,
-                       c.count_rays != 0, &rs);
+                       tune ? tune->group_side : c.wavefront_group, c.count_rays != 0, &rs);
     if (rc == PRT_OK) {
         for (uint32_t i = 0; i < n && rc == PRT_OK; ++i) {
             if (!jobs[i].out_rgb) continue;
@@ -1178,11 +1232,16 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     int rc = set_device(c.device);
     if (rc != PRT_OK) return rc;
     PRT_TRY_BEGIN
-    // Trial workload: every job at min(spp, 4) samples per pixel.
+    // Trial workload: every job at its own samples per pixel and width, but with only
+    // kTrialRows rows (the camera's vertical field kept, so the rows spread over the
+    // whole view): the camera-path grouping, whose candidates depend on the sample
+    // count and the row count, then behaves as in the full run, at a small fraction
+    // of its paths.
+    constexpr int32_t kTrialRows = 8;
     std::vector<prt_job> trial(jobs, jobs + n);
     uint64_t paths = 0;
     for (auto& t : trial) {
-        t.spp = std::min<uint32_t>(t.spp, 4);
+        t.camera.ypixels = std::min<int32_t>(t.camera.ypixels, kTrialRows);
         t.out_rgb = nullptr;
         paths += uint64_t(t.camera.xpixels) * t.camera.ypixels * t.spp;
     }
@@ -1199,28 +1258,60 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
     WavefrontState w;
     double bms = 1e30;
     prt_tune bt{};
-    // Queue depth: the fewest chunks of the full work that fit the path slots (larger
-    // chunks were faster at every size measured, since every chunk ends in nearly empty
-    // iterations). Occupancy: timed on the trial, cut into chunks of that same size
-    // (or one chunk when the trial is smaller), so each trial chunk behaves like a full
-    // one. The candidates stay near full occupancy: wf_extend fits 32 workgroups of one
-    // wave per CU (8 per SIMD), wf_shadow 28.
+    // Every candidate is timed (best of two) on the trial cut into as many chunks as
+    // the full run at that chunk size would take. Occupancy candidates stay near full
+    // occupancy: wf_extend fits 32 one-wave workgroups per CU (8 per SIMD), wf_shadow
+    // 28. Chunk counts: the fewest that fit the path slots (larger chunks were faster
+    // at every size measured on one view, since every chunk ends in nearly empty
+    // iterations), and twice and four times as many.
     const uint64_t limit = max_slots();
     const uint32_t full_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full, limit)));
-    const uint32_t trial_cap = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(full_cap, paths)));
-    const uint32_t bpcs[] = {20, 24, 28, 32};
-    for (uint32_t bpc : bpcs) {
-        double t = 1e30;
+    // The trial's path slots for a full-run chunk size: the trial is cut into as many
+    // chunks as the full run would take, so its queues drain like the full run's.
+    auto trial_cap_for = [&](uint32_t cap) {
+        const uint64_t chunks = (full + cap - 1) / cap;
+        return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(paths, (paths + chunks - 1) / chunks)));
+    };
+    auto time_trial = [&](const prt_tune& t) -> double {
+        double best_t = 1e30;
         for (int rep = 0; rep < 2 && rc == PRT_OK; ++rep) {  // best of two
             RunStats rs;
-            rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, trial_cap, bpc, false, &rs);
-            t = std::min(t, rs.ms);
+            rc = run_wavefront(w, c.device, s, b.jobs, b.pjobs, c.max_depth, trial_cap_for(t.queue_slots),
+                               t.blocks_per_cu, t.group_side, false, &rs);
+            best_t = std::min(best_t, rs.ms);
         }
-        if (rc != PRT_OK) break;
-        if (t < bms) {
-            bms = t;
-            bt = prt_tune{full_cap, 0, kBlock, bpc};
+        return best_t;
+    };
+    // A candidate replaces the best so far only when it is at least 2% faster on the
+    // trial: smaller differences are within the trial's run-to-run spread and need
+    // not carry over to the full run.
+    auto consider = [&](const prt_tune& t) {
+        const double ms = time_trial(t);
+        if (rc == PRT_OK && (bms >= 1e30 || ms < 0.98 * bms)) {
+            bms = ms;
+            bt = t;
         }
+    };
+    // Coordinate search from the defaults (fewest chunks, smallest dividing group):
+    // occupancy, then the camera-path grouping, then the chunk count.
+    bt = prt_tune{full_cap, 0, kBlock, 0};
+    consider(bt);
+    for (uint32_t bpc : {20u, 24u, 28u}) {
+        prt_tune t = bt;
+        t.blocks_per_cu = bpc;
+        if (rc == PRT_OK) consider(t);
+    }
+    const prt_tune occ = bt;
+    for (uint32_t g : {1u, 2u, 4u, 8u}) {
+        prt_tune t = occ;
+        t.group_side = g;
+        if (rc == PRT_OK) consider(t);
+    }
+    const prt_tune grp = bt;
+    for (uint32_t div : {2u, 4u}) {
+        prt_tune t = grp;
+        t.queue_slots = std::max<uint32_t>(1u << 16, full_cap / div);
+        if (t.queue_slots < full_cap && rc == PRT_OK) consider(t);
     }
     free_state(w);
     (void)hipStreamDestroy(s);

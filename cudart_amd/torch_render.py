@@ -32,11 +32,21 @@ class PathTrace(torch.autograd.Function):
         ctx.spp = spp
         ctx.hess = hess
         ctx.devices = (albedo.device, emission.device)
+        ctx.params = (kd.copy(), em.copy())
         return img
 
     @staticmethod
     def backward(ctx, grad_img: torch.Tensor):
         g = np.ascontiguousarray(grad_img.detach().to("cpu", torch.float32).numpy())
+        # The gradient is taken at the parameters this node's forward rendered with:
+        # when another forward call has set other ones since, they are set back (the
+        # renderer's kept radiance then no longer matches the scene's materials, and
+        # the adjoint traces each sample's primal path at these parameters again).
+        scene = ctx.renderer.scene
+        kd, em = ctx.params
+        if not scene.has_material_params(kd, em):
+            for m in range(kd.shape[0]):
+                scene.set_material_params(m, kd[m], float(em[m]))
         d_alb, d_em = ctx.renderer.adjoint_grad(g, ctx.spp, ctx.hess)
         return (torch.from_numpy(d_alb).to(ctx.devices[0]), torch.from_numpy(d_em).to(ctx.devices[1]), None, None,
                 None)

@@ -117,7 +117,9 @@ typedef struct prt_render_config {
     uint32_t integrator;    /* PRT_INTEGRATOR_* */
     uint32_t spp_per_launch;/* megakernel: samples per pixel per launch (0 = choose) */
     uint32_t count_rays;    /* 1 = keep per-kind ray / node / triangle counters */
-    uint32_t wavefront_chunk;   /* wavefront: samples per pixel resolved per chunk (0 = choose) */
+    uint32_t wavefront_group;   /* wavefront: side g (1, 2, 4 or 8) of the pixel block whose
+                                   64 / g^2 consecutive samples make one wave's 64 camera paths
+                                   (0 = the smallest g that divides the tile) */
     uint32_t wavefront_queue;   /* wavefront: path slots in flight (0 = choose) */
     uint32_t profile_kernels;   /* wavefront: 1 = launch each chunk's kernels without a graph and
                                    time every extend / shade / shadow launch with events */
@@ -281,10 +283,12 @@ typedef struct prt_job {
 
 /* Tunables picked by prt_autotune. Zero fields mean "choose". */
 typedef struct prt_tune {
-    uint32_t queue_slots;   /* paths in flight */
-    uint32_t chunk_spp;     /* samples per pixel per resolve chunk */
-    uint32_t block_size;    /* threads per workgroup of the queue kernels */
-    uint32_t blocks_per_cu; /* persistent workgroups per compute unit */
+    uint32_t queue_slots;   /* path slots per chunk (the batch runs in ceil(paths / slots) chunks) */
+    uint32_t group_side;    /* camera-path grouping of wf_generate: side g (1, 2, 4, 8) of the pixel
+                               block of one wave's 64 paths (as prt_render_config.wavefront_group) */
+    uint32_t block_size;    /* threads per workgroup of the traversal kernels: only 64 (one wave)
+                               is built; 0 or 64 are accepted, anything else is PRT_ERR_INVALID_ARG */
+    uint32_t blocks_per_cu; /* persistent traversal workgroups per compute unit */
 } prt_tune;
 
 /* Renders every job in one persistent wavefront launch sequence (a captured HIP
@@ -293,7 +297,12 @@ typedef struct prt_tune {
 int prt_render_batch(prt_job *jobs, uint32_t num_jobs, const prt_render_config *cfg,
                      const prt_tune *tune, prt_stats *stats);
 
-/* Times candidate tunings on the given jobs and writes the fastest into best. */
+/* Times candidate tunings on a trial of the given jobs (each view at its own width
+ * and samples per pixel over 8 rows spread across its field, cut into as many
+ * chunks as the full run would take) and writes the fastest into best: traversal
+ * occupancy (blocks_per_cu 20..32), then the camera-path grouping (g = 1, 2, 4, 8),
+ * then the chunk count (the fewest chunks that fit, twice and four times as many),
+ * each searched with the others held at their best so far. best_ms = the trial's time. */
 int prt_autotune(prt_job *jobs, uint32_t num_jobs, const prt_render_config *cfg, prt_tune *best,
                  double *best_ms);
 

@@ -17,6 +17,15 @@
  *   - Light list in original triangle order; total light area summed in that order.
  *   - Tonemap clamps negative/NaN to 0 (Q12).
  * Everything else, including the estimator quirks Q1-Q11, follows the reference.
+ *
+ * How it is pinned: tests/test_oracle_kat.py checks every part against published
+ * vectors (Philox), closed forms and whole-render known answers in which each
+ * branch of samplePixel (emission, DIFFSPEC, REFRACTIVE with the Fresnel quirk,
+ * DIFFUSE + NEE, SPECULAR, Russian roulette) has a value fixed by the reference's
+ * semantics; tests/test_oracle_mutations.py plants mistakes in copies of this file
+ * and requires those checks to catch each one. Parity unpinned: the reference's own
+ * random streams, texture filtering and fast-math intrinsics, which are replaced
+ * (nondeterministic or undefined in the reference).
  */
 #include <omp.h>
 

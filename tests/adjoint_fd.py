@@ -3,8 +3,8 @@
 replay, L2 loss against a target rendered from parameters perturbed by +-0.1).
 
 For every seed the image, the adjoint and the finite differences (h = 1e-3) use the
-same random numbers; the target is rendered at `target_spp` samples from another
-seed, like a reference photograph independent of the image's noise. Central
+same random numbers; the target is rendered once, at `target_spp` samples from a
+seed no image uses, like a reference photograph independent of the image's noise. Central
 differences also see the Russian-roulette decisions that flip inside +-h; their
 contribution is unbiased but noisy (variance ~ 1 / (paths * h)), so the comparison
 is of the means over seeds, and the error of the mean falls as 1/sqrt(seeds) until
@@ -41,9 +41,11 @@ def _materials(wl, seed=1):
     return kd0, em0, kdT, emT
 
 
-def compare(seeds, w=512, h=512, spp=64, target_spp=512, log=print):
+def compare(seeds, w=512, h=512, spp=64, target_spp=512, rr_start=3, integrator=ca.INTEGRATOR_WAVEFRONT,
+            log=print):
     """Returns (relative L2 error of the mean adjoint vs the mean FD after each seed,
-    mean adjoint, mean FD, mean first-order adjoint)."""
+    mean adjoint, mean FD, mean first-order adjoint, relative L2 error of each seed).
+    rr_start >= the depth (6) renders without Russian roulette."""
     wl = W.cornell100k(w, h)
     scene = ca.Scene(wl.mesh)
     kd0, em0, kdT, emT = _materials(wl)
@@ -53,17 +55,21 @@ def compare(seeds, w=512, h=512, spp=64, target_spp=512, log=print):
         for i in range(M):
             scene.set_material_params(i, kd[i], float(em[i]))
 
+    # One target for every seed: a photograph of the perturbed scene, rendered from
+    # random numbers no image below uses.
+    kw0 = dict(camera=wl.camera, max_depth=wl.max_depth, rr_start=rr_start, integrator=integrator)
+    rt = ca.ParallelRenderer(scene, w, h, target_spp, seed=0x7A5C3E1, **kw0)
+    set_params(kdT, emT)
+    rt.render(target_spp)
+    target = rt.radiance().astype(np.float64)
+    rt.close()
+    set_params(kd0, em0)
+
     g_sum = fd_sum = g1_sum = None
-    errs = []
+    errs, each = [], []
     for k, sd in enumerate(seeds):
         t0 = time.time()
-        kw = dict(camera=wl.camera, max_depth=wl.max_depth, seed=sd, spp_per_launch=16)
-        rt = ca.ParallelRenderer(scene, w, h, target_spp, **{**kw, "seed": sd ^ 0x7A5C3E1})
-        set_params(kdT, emT)
-        rt.render(target_spp)
-        target = rt.radiance().astype(np.float64)
-        rt.close()
-        r = ca.ParallelRenderer(scene, w, h, spp, **kw)
+        r = ca.ParallelRenderer(scene, w, h, spp, seed=sd, **kw0)
 
         def loss_at(kd, em):
             set_params(kd, em)
@@ -95,14 +101,15 @@ def compare(seeds, w=512, h=512, spp=64, target_spp=512, log=print):
         g_sum = g if g_sum is None else g_sum + g
         g1_sum = g1 if g1_sum is None else g1_sum + g1
         fd_sum = fd if fd_sum is None else fd_sum + fd
+        each.append(float(np.linalg.norm(g - fd) / np.linalg.norm(fd)))
         errs.append(float(np.linalg.norm(g_sum - fd_sum) / np.linalg.norm(fd_sum)))
-        log(f"seed {k + 1}/{len(seeds)} ({sd}): this seed {np.linalg.norm(g - fd) / np.linalg.norm(fd):.4f}, "
+        log(f"seed {k + 1}/{len(seeds)} ({sd}): this seed {each[-1]:.4f}, "
             f"mean so far {errs[-1]:.4f}, first-order mean "
             f"{np.linalg.norm(g1_sum - fd_sum) / np.linalg.norm(fd_sum):.4f} ({time.time() - t0:.1f} s)")
         sys.stdout.flush()
     n = len(seeds)
     scene.close()
-    return errs, g_sum / n, fd_sum / n, g1_sum / n
+    return errs, g_sum / n, fd_sum / n, g1_sum / n, each
 
 
 def seed_list(n):
@@ -113,8 +120,9 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, default=48)
     ap.add_argument("--target-spp", type=int, default=512)
+    ap.add_argument("--rr-start", type=int, default=3, help="6 or more: no Russian roulette")
     args = ap.parse_args()
-    errs, g, fd, g1 = compare(seed_list(args.seeds), target_spp=args.target_spp)
+    errs, g, fd, g1, _ = compare(seed_list(args.seeds), target_spp=args.target_spp, rr_start=args.rr_start)
     print(f"relative L2 error of the mean adjoint vs the mean central FD over {args.seeds} seeds: {errs[-1]:.5f}")
     print(f"first-order adjoint (no roulette curvature term): {np.linalg.norm(g1 - fd) / np.linalg.norm(fd):.5f}")
     big = np.abs(fd) >= 0.1 * np.abs(fd).max()

@@ -80,10 +80,25 @@ def test_autotune_returns_a_working_tuning(gpu):
     s = ca.Scene(a.mesh)
     jobs = [ca.BatchJob(s, a.camera, 4)]
     tune, ms = ca.autotune(jobs, max_depth=5)
-    assert tune.queue_slots > 0 and tune.blocks_per_cu > 0 and ms > 0
+    assert tune.queue_slots > 0 and tune.block_size == 64 and ms > 0
+    assert tune.group_side in (0, 1, 2, 4, 8) and tune.blocks_per_cu in (0, 20, 24, 28)
     imgs, _ = ca.render_batch(jobs, tune=tune, max_depth=5)
     ref, _ = _render(s, a.camera, 64, 64, 4, max_depth=5)
     assert np.array_equal(_bits(imgs[0]), _bits(ref))
+    # every forced camera-path grouping and a small chunk size give the same image
+    for g in (1, 2, 4, 8):
+        t = ca.Tune()
+        t.queue_slots, t.group_side = 4096, g
+        imgs, _ = ca.render_batch(jobs, tune=t, max_depth=5)
+        assert np.array_equal(_bits(imgs[0]), _bits(ref)), g
+    bad = ca.Tune()
+    bad.group_side = 3
+    with pytest.raises(ca.PrtError):
+        ca.render_batch(jobs, tune=bad, max_depth=5)
+    bad = ca.Tune()
+    bad.block_size = 128
+    with pytest.raises(ca.PrtError):
+        ca.render_batch(jobs, tune=bad, max_depth=5)
 
 
 def test_deep_stack_spill_equals_megakernel_and_oracle(gpu):
