@@ -28,10 +28,11 @@ achieved / peak. Beside it: the kernel's algorithmic bytes per launch
     + 48 B x hits (the winning record re-read for u, v, material)
     + 52 B x rays (32 B queue entry read, 20 B hit record written)
 (device counters of the instrumented frame, shadow rays removed), most of which
-the L2 and the MALL serve; and the instruction-issue fraction: the kernel's wave64
-VALU instructions per launch (committed SQ_INSTS_VALU summary) per second of the
+the L2 and the MALL serve; and the instruction-issue fraction: the kernel's VALU
+work per launch (committed SQ counter summary) per second of the
 live launch time, over the rate a calibration kernel of independent fp32 FMAs
-reaches on the same GPU (bench.py --issue-probe, committed summary).
+reaches on the same GPU (bench.py --issue-probe, committed summary), counted in
+VALU lane operations (SQ_THREAD_CYCLES_VALU: instructions times active lanes).
 
 Adjoint leg: path-replay gradients of the L2 image loss w.r.t. 16 materials'
 albedo and emission on the 100k-triangle Cornell-like scene at 512x512, 64
@@ -380,15 +381,16 @@ def main():
         roofline.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4),
                         basis="measured HBM bytes per launch (profiles/) / live launch time",
                         traffic_over_algorithmic=round(traffic / bytes_per_launch, 4))
-        valu = summary_value(ISSUE_FILE, "wf_extend_valu_instructions_per_launch")
-        peak = summary_value(ISSUE_FILE, "issue_probe_valu_instructions_per_s")
+        lane_ops = summary_value(ISSUE_FILE, "wf_extend_valu_lane_ops_per_launch")
+        peak = summary_value(ISSUE_FILE, "issue_probe_valu_lane_ops_per_s")
         lanes = summary_value(ISSUE_FILE, "wf_extend_active_lanes_per_valu")
-        if valu and peak:
-            rate = valu / (launch_ms * 1e-3)
-            roofline["issue"] = {"achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
-                                 "unit": "G wave64 VALU instructions/s", "frac": round(rate / peak, 4),
+        if lane_ops and peak:
+            rate = lane_ops / (launch_ms * 1e-3)
+            roofline["issue"] = {"achieved": round(rate / 1e12, 2), "peak": round(peak / 1e12, 2),
+                                 "unit": "T VALU lane operations/s", "frac": round(rate / peak, 4),
                                  "active_lanes_per_valu": lanes,
-                                 "peak_source": "calibration kernel of independent fp32 FMAs on the same GPU "
+                                 "peak_source": "calibration kernel of independent fp32 FMAs, all lanes active, on the "
+                                                "same GPU: one wave64 VALU instruction per CU per cycle "
                                                 "(bench.py --issue-probe; profiles/)"}
     else:
         # No counter summary for this configuration: the algorithmic rate stands in.

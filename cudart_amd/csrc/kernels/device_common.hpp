@@ -213,9 +213,6 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
 // planes are the values min / max would give. i is finite and nonzero
 // (slab_inverse), so no plane is NaN for a finite origin.
 constexpr int kNoChild = 0x7FFFFFFF;  // also the word of an empty slot
-#ifndef PRT_FOLD_DECODE
-#define PRT_FOLD_DECODE 0
-#endif
 __device__ __forceinline__ void cswap(float& ka, int& wa, float& kb, int& wb) {
     const bool sw = kb < ka;
     const float k = sw ? kb : ka;
@@ -242,22 +239,8 @@ __device__ __forceinline__ int visit_node(const float4* nodes, int node, float i
     const uint32_t nqx = fx ? qxy.y : qxy.x, fqx = fx ? qxy.x : qxy.y;
     const uint32_t nqy = fy ? qxy.w : qxy.z, fqy = fy ? qxy.z : qxy.w;
     const uint32_t nqz = fz ? qz.y : qz.x, fqz = fz ? qz.x : qz.y;
-#if PRT_FOLD_DECODE
-    // t of plane q: (q s + org - o) i = q (s i) + (org i - o i); s = 2^e makes s i
-    // exact, so one FMA per plane (the rounding differs from decode-then-slab by an
-    // ulp of t, far inside the boxes' padding).
-    const float ax = sx * ix, ay = sy * iy, az = sz * iz;
-    const float bx = __builtin_fmaf(org.x, ix, -ox), by = __builtin_fmaf(org.y, iy, -oy),
-                bz = __builtin_fmaf(org.z, iz, -oz);
-#endif
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-#if PRT_FOLD_DECODE
-        const float tn = fmaxf(fmaxf(__builtin_fmaf(qbyte(nqx, c), ax, bx), __builtin_fmaf(qbyte(nqy, c), ay, by)),
-                               fmaxf(__builtin_fmaf(qbyte(nqz, c), az, bz), tmin));
-        const float tf = fminf(fminf(__builtin_fmaf(qbyte(fqx, c), ax, bx), __builtin_fmaf(qbyte(fqy, c), ay, by)),
-                               fminf(__builtin_fmaf(qbyte(fqz, c), az, bz), tfar));
-#else
         const float nx = __builtin_fmaf(qbyte(nqx, c), sx, org.x), fx_ = __builtin_fmaf(qbyte(fqx, c), sx, org.x);
         const float ny = __builtin_fmaf(qbyte(nqy, c), sy, org.y), fy_ = __builtin_fmaf(qbyte(fqy, c), sy, org.y);
         const float nz = __builtin_fmaf(qbyte(nqz, c), sz, org.z), fz_ = __builtin_fmaf(qbyte(fqz, c), sz, org.z);
@@ -265,7 +248,6 @@ __device__ __forceinline__ int visit_node(const float4* nodes, int node, float i
                                fmaxf(__builtin_fmaf(nz, iz, -oz), tmin));
         const float tf = fminf(fminf(__builtin_fmaf(fx_, ix, -ox), __builtin_fmaf(fy_, iy, -oy)),
                                fminf(__builtin_fmaf(fz_, iz, -oz), tfar));
-#endif
         const bool hit = tn <= tf && word[c] != kNoChild;
         key[c] = hit ? tn : __builtin_inff();
         word[c] = hit ? word[c] : kNoChild;

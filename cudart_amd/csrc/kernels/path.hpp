@@ -31,7 +31,6 @@ struct NeeSample {
     V3 mask;         // throughput after this vertex's diffuse weight
     uint32_t li;     // light record
     float lu, lv;    // point on the light (already folded when lu + lv >= 1)
-    uint32_t pick;   // position of the light in the scene's light list
 };
 
 struct ShadowReq {
@@ -186,8 +185,7 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
         // Light choice (renderer.cu:218-221); estimateDirectLighting returns 0 when the
         // shading triangle is the chosen light (:421-423).
         float pick = u01(ra.w) * (float(sc.num_lights) - .00001f);
-        const uint32_t pi = uint32_t(truncf(pick));
-        const uint32_t li = sc.lights[pi];
+        const uint32_t li = sc.lights[uint32_t(truncf(pick))];
         if (h.rec != li) {
             float lu = u01(rb.x), lv = u01(rb.y);
             if (lu + lv >= 1.0f) {
@@ -200,7 +198,6 @@ __device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs
             ns.li = li;
             ns.lu = lu;
             ns.lv = lv;
-            ns.pick = pi;
             *want_nee = true;
         } else {
             ps.color = ps.color + ps.mask * v3(0.f, 0.f, 0.f);

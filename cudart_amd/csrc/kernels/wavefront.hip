@@ -276,6 +276,8 @@ static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase th
 #ifndef PRT_EXTEND_WAVES
 #define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(8)))
 #endif
+// wf_shade keeps the compiler's occupancy (104 VGPRs, 4 waves per SIMD): forcing 5
+// or 6 waves spills 36 / 104 bytes per lane and measured 2% / 12% slower per frame.
 #ifndef PRT_SHADE_WAVES
 #define PRT_SHADE_WAVES
 #endif
@@ -481,61 +483,39 @@ __global__ __launch_bounds__(kBlock) PRT_EXTEND_WAVES void wf_extend(WfArgs a, u
     if (COUNT) flush_stats<false>(a.ctr, cnt);
 }
 
-// Shadow-queue classes: a workgroup's shadow rays are appended grouped by the part
-// of the scene's light list their light sample lies in (class = position * K /
-// number of lights), so the 64-ray batches the occlusion kernel fetches mostly aim
-// at one light. With several emitters far apart (two 4 x 4 quads 10 units apart in
-// the 1M-triangle scene), the rays of one shading point otherwise split between
-// subtrees and the lanes of a wave part ways. The order of the shadow queue does not
-// change any result: each entry updates only its own path.
-#ifndef PRT_SHADOW_CLASSES
-#define PRT_SHADOW_CLASSES 4
-#endif
-constexpr uint32_t kShadowClasses = PRT_SHADOW_CLASSES;
-static_assert(kShadowClasses >= 1 && kShadowClasses <= 8, "shadow classes out of range");
-
 // Appends to two queues for a whole workgroup with one atomic per queue: lanes
-// with more get consecutive next-queue positions, lanes with nee consecutive
-// shadow positions ordered by shadow class (then wave, then lane). Returns
-// (next-queue, shadow) positions.
-__device__ __forceinline__ uint2 block_append(bool more, bool nee, uint32_t cls, uint32_t* ctr_next,
-                                              uint32_t* ctr_shadow) {
-    constexpr uint32_t kWaves = kShadeBlock / 64;
-    __shared__ uint32_t s_off0[kWaves];
-    __shared__ uint32_t s_off1[kShadowClasses][kWaves];
+// with more / nee get consecutive positions. Returns (next-queue, shadow) positions.
+// (Grouping a workgroup's shadow rays by the light they aim at, so that 64-ray
+// batches mostly share a light, measured 2% slower per frame on the 1M-triangle
+// scene with its two lights 10 units apart: the queue order of shade, pixel by
+// pixel, is the better one for the L2.)
+__device__ __forceinline__ uint2 block_append(bool more, bool nee, uint32_t* ctr_next, uint32_t* ctr_shadow) {
+    __shared__ uint32_t s_off[2][kShadeBlock / 64];
     __shared__ uint32_t s_base[2];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    const unsigned long long m0 = __ballot(more);
-    unsigned long long mine = 0ull;  // this lane's class mask
-#pragma unroll
-    for (uint32_t c = 0; c < kShadowClasses; ++c) {
-        const unsigned long long mc = __ballot(nee && cls == c);
-        if (cls == c) mine = mc;
-        if (lane == 0) s_off1[c][wave] = uint32_t(__popcll(mc));
+    const unsigned long long m0 = __ballot(more), m1 = __ballot(nee);
+    if (lane == 0) {
+        s_off[0][wave] = uint32_t(__popcll(m0));
+        s_off[1][wave] = uint32_t(__popcll(m1));
     }
-    if (lane == 0) s_off0[wave] = uint32_t(__popcll(m0));
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t0 = 0, t1 = 0;
-        for (uint32_t w = 0; w < kWaves; ++w) {
-            const uint32_t c0 = s_off0[w];
-            s_off0[w] = t0;
+        for (int w = 0; w < kShadeBlock / 64; ++w) {
+            uint32_t c0 = s_off[0][w], c1 = s_off[1][w];
+            s_off[0][w] = t0;
+            s_off[1][w] = t1;
             t0 += c0;
+            t1 += c1;
         }
-        for (uint32_t c = 0; c < kShadowClasses; ++c)
-            for (uint32_t w = 0; w < kWaves; ++w) {
-                const uint32_t c1 = s_off1[c][w];
-                s_off1[c][w] = t1;
-                t1 += c1;
-            }
         s_base[0] = t0 ? atomicAdd(ctr_next, t0) : 0u;
         s_base[1] = t1 ? atomicAdd(ctr_shadow, t1) : 0u;
     }
     __syncthreads();
     const unsigned long long lt = (1ull << lane) - 1ull;
     uint2 r;
-    r.x = s_base[0] + s_off0[wave] + uint32_t(__popcll(m0 & lt));
-    r.y = nee ? s_base[1] + s_off1[cls][wave] + uint32_t(__popcll(mine & lt)) : 0u;
+    r.x = s_base[0] + s_off[0][wave] + uint32_t(__popcll(m0 & lt));
+    r.y = s_base[1] + s_off[1][wave] + uint32_t(__popcll(m1 & lt));
     __syncthreads();  // s_off / s_base are reused by the next iteration
     return r;
 }
@@ -596,12 +576,7 @@ __global__ __launch_bounds__(kShadeBlock) PRT_SHADE_WAVES void wf_shade(WfArgs a
             }
         });
         if (valid && !more) a.result[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.f);
-        uint32_t cls = 0;
-        if (want_nee && kShadowClasses > 1) {
-            const uint32_t nl = a.jobs[pj >> 24].sc.num_lights;
-            cls = uint32_t((uint64_t(ns.pick) * kShadowClasses) / nl);
-        }
-        const uint2 pos = block_append(more, want_nee, cls, &a.ctr[kExt0 + (q ^ 1u)], &a.ctr[kShadow]);
+        const uint2 pos = block_append(more, want_nee, &a.ctr[kExt0 + (q ^ 1u)], &a.ctr[kShadow]);
         const uint32_t qi = pos.x;
         if (more) {
             a.qa[q ^ 1u][qi] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, A.w);

@@ -30,3 +30,46 @@ def test_workload_defaults(monkeypatch, workload, expect):
     monkeypatch.setattr(sys, "argv", ["bench.py", "--workload", workload, "--spp", "4", "--depth", "3"])
     a = bench.parse()
     assert (a.spp, a.depth) == (4, 3)
+
+
+def test_cpu_baseline_uses_every_allowed_core_and_summaries_parse(tmp_path):
+    sys.path.insert(0, str(ROOT))
+    import os
+
+    import bench
+
+    assert bench.host_cores() == len(os.sched_getaffinity(0))
+    assert isinstance(bench.cpu_model(), str) and bench.cpu_model()
+    f = tmp_path / "summary.txt"
+    f.write_text("# comment\nalpha = 12.5\nbeta=3e9\n")
+    assert bench.summary_value(f, "alpha") == 12.5
+    assert bench.summary_value(f, "beta") == 3e9
+    assert bench.summary_value(f, "gamma") is None
+    assert bench.summary_value(tmp_path / "missing.txt", "alpha") is None
+    # the committed summaries the default bench line reads carry their keys
+    assert bench.summary_value(bench.TRAFFIC_FILE, "traffic_bytes_per_launch") > 1e9
+    assert bench.summary_value(bench.ISSUE_FILE, "wf_extend_valu_lane_ops_per_launch") > 1e10
+    assert bench.summary_value(bench.ISSUE_FILE, "issue_probe_valu_lane_ops_per_s") > 1e12
+
+
+def test_rmse_reference_fixture_and_metric():
+    sys.path.insert(0, str(ROOT))
+    import numpy as np
+
+    import bench
+    from cudart_amd import workloads as W
+
+    ref = np.load(bench.RMSE_FILE, allow_pickle=False)
+    assert ref["rgb"].shape == (len(ref["pixels"]), 3) and int(ref["spp"]) == 8192
+    assert (int(ref["width"]), int(ref["height"])) == (1920, 1080)
+
+    class Wl:  # the default frame's geometry, without building the scene
+        width, height, max_depth = 1920, 1080, 8
+
+    img = np.zeros((1080, 1920, 3), np.float32)
+    img.reshape(-1, 3)[ref["pixels"]] = ref["rgb"].astype(np.float32)
+    r = bench.rmse_against_reference(img, Wl)
+    assert r["value"] < 1e-6 and r["pixels"] == len(ref["pixels"])
+    img.reshape(-1, 3)[ref["pixels"]] += np.float32(0.01)
+    assert abs(bench.rmse_against_reference(img, Wl)["value"] - 0.01) < 1e-6
+    assert W.SEED == int(ref["seed"])
