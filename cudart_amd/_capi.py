@@ -123,6 +123,7 @@ SIGNATURES = {
     "prt_autotune": (I, [C.POINTER(Job), U, C.POINTER(RenderConfig), C.POINTER(Tune), C.POINTER(C.c_double)]),
     "prt_adjoint": (I, [P, FP, FP, U, FP, FP, FP]),
     "prt_adjoint_grad": (I, [P, FP, C.c_float, U, FP, FP]),
+    "prt_adjoint_grad_static": (I, [P, C.c_void_p, C.c_float, U, FP, FP]),
     "prt_write_png": (I, [C.c_char_p, U, U, BP]),
     "prt_write_pfm": (I, [C.c_char_p, U, U, FP]),
 }

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../host/common.hpp"
@@ -409,6 +410,36 @@ __global__ __launch_bounds__(kReduceBlock) void adjoint_reduce(const double* par
 
 }  // namespace dev
 
+// Scratch of prt_adjoint_grad_static, the entry point that allocates nothing: the
+// per-workgroup partial gradients (grid x P doubles), then the P reduced
+// gradients. 16 MiB on every device the library runs that entry point on, held
+// until the process ends. In host code `scratch` names a host-side stand-in, not
+// device memory: host code uses only the address hipGetSymbolAddress returns.
+constexpr size_t kStaticScratchDoubles = size_t(1) << 21;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device` (the current device), looked up once
+// per device on that device's first call.
+int static_scratch(int device, double** out) {
+    if (device < 0 || device >= kMaxScratchDevices) return fail(PRT_ERR_LIMIT, "device ordinal above 63");
+    std::lock_guard<std::mutex> g(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(PRT_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return PRT_OK;
+}
+}  // namespace
+
 namespace {
 #define ADJ_TRY(expr)                                           \
     do {                                                        \
@@ -420,6 +451,11 @@ namespace {
     } while (0)
 // PRT_ADJOINT_PRIMAL_IN_REPLAY=1: without kept radiance, trace each sample's primal
 // path inside the replay kernel even where the per-sample buffer would fit (tests).
+#define HIP_CHECK(expr)                                    \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);  \
+    } while (0)
 bool primal_in_replay() {
     const char* v = std::getenv("PRT_ADJOINT_PRIMAL_IN_REPLAY");
     return v && v[0] == '1';
@@ -528,6 +564,56 @@ done:
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     return rc;
+}
+
+extern "C" int prt_adjoint_grad_static(prt_renderer* r, const float* grad_rgb_device, float hess, uint32_t spp,
+                                       float* d_albedo, float* d_emission) {
+    clear_error();
+    if (!r || !grad_rgb_device || !d_albedo || !d_emission || spp == 0 || !(hess >= 0.f))
+        return fail(PRT_ERR_INVALID_ARG, "bad argument");
+    if (r->cfg.max_depth > 16) return fail(PRT_ERR_LIMIT, "the adjoint supports max_depth <= 16");
+    int rc = set_device(r->cfg.device);
+    if (rc != PRT_OK) return rc;
+    {
+        hipPointerAttribute_t at;
+        hipError_t e = hipPointerGetAttributes(&at, grad_rgb_device);
+        if (e != hipSuccess) (void)hipGetLastError();
+        if (e != hipSuccess || at.type != hipMemoryTypeDevice || at.device != r->cfg.device)
+            return fail(PRT_ERR_INVALID_ARG, "grad_rgb_device is not device memory of the renderer's device");
+    }
+    // The renderer's creation uploaded the scene to this device: this refreshes at
+    // most the material table in place (a copy, no allocation).
+    rc = device_scene(r->scene, r->cfg.device, r->sc);
+    if (rc != PRT_OK) return rc;
+    const uint32_t M = r->scene->num_materials;
+    const uint32_t P = 4 * M;
+    const uint32_t grid = ((r->tiles + 7) / 8) * 8;
+    if (size_t(grid) * P + P > kStaticScratchDoubles)
+        return fail(PRT_ERR_LIMIT, "image too large for the static scratch: (8x8 tiles rounded up to 8) x 4 x "
+                                   "materials + 4 x materials must be at most 2^21 doubles");
+    double* base = nullptr;
+    rc = static_scratch(r->cfg.device, &base);
+    if (rc != PRT_OK) return rc;
+    // Every workgroup writes all P of its partials and the reduction writes all P
+    // results, so nothing in the scratch needs clearing between calls.
+    AdjArgs a = adjoint_args(r, grad_rgb_device, hess, spp, base);
+    double* d_out = base + size_t(grid) * P;
+    double g[4 * PRT_MAX_MATERIALS];
+    float ms = 0.f;
+    HIP_CHECK(hipEventRecord(r->ev0, r->stream));
+    hipLaunchKernelGGL(dev::adjoint_kernel, dim3(grid), dim3(kBlock), 0, r->stream, a);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(dev::adjoint_reduce, dim3(P), dim3(dev::kReduceBlock), 0, r->stream,
+                       static_cast<const double*>(base), grid, P, d_out);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(r->ev1, r->stream));
+    HIP_CHECK(hipEventSynchronize(r->ev1));
+    HIP_CHECK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    HIP_CHECK(hipMemcpy(g, d_out, P * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < 3 * M; ++k) d_albedo[k] = float(g[k]);
+    for (uint32_t k = 0; k < M; ++k) d_emission[k] = float(g[3 * M + k]);
+    r->last_ms = ms;
+    return PRT_OK;
 }
 
 extern "C" int prt_adjoint(prt_renderer* r, const float* target_rgb, const float* image_rgb, uint32_t spp,

@@ -328,6 +328,21 @@ int prt_adjoint(prt_renderer *r, const float *target_rgb, const float *image_rgb
 int prt_adjoint_grad(prt_renderer *r, const float *grad_rgb, float hess, uint32_t spp, float *d_albedo,
                      float *d_emission);
 
+/* prt_adjoint_grad for callers that cannot allocate (code under HIP graph capture,
+ * tight optimisation or serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. It calls no hipMalloc / hipFree: grad_rgb_device is
+ * the caller's device memory on the renderer's device (W*H*3 floats, dLoss/dI), the
+ * per-workgroup partial gradients live in a static 16 MiB device array (2^21
+ * doubles) that each GPU holds until the process ends, and it never writes a
+ * per-sample radiance buffer: without radiance kept by the forward pass
+ * (keep_samples) each sample's primal path is traced inside the replay. Limit:
+ * ceil8(ceil(W/8) * ceil(H/8)) * 4M + 4M <= 2^21 (e.g. 1920x1080 with 16 materials);
+ * larger calls fail with PRT_ERR_LIMIT. The gradients equal prt_adjoint_grad's for
+ * the same inputs. Calls that overlap on one GPU (several host threads) share its
+ * array and race; serialise them. d_albedo: M*3, d_emission: M (host, overwritten). */
+int prt_adjoint_grad_static(prt_renderer *r, const float *grad_rgb_device, float hess, uint32_t spp,
+                            float *d_albedo, float *d_emission);
+
 /* Changes material parameters of a built scene in place (re-uploaded on next render). */
 int prt_scene_set_material_params(prt_scene *scene, uint32_t index, const float kd[3], float emission);
 
