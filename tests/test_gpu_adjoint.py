@@ -4,7 +4,7 @@ Checks: (1) equality with the oracle's replay of the same fp32 paths (only the
 fp64 summation order differs); (2) central finite differences (h = 1e-3, common
 random numbers) of the GPU forward renders: without Russian roulette to 1e-3
 (relative L2 of the whole gradient vector), and at the full 512x512 64-sample
-configuration within 1% on every seed; with roulette, the mean over 24 seeds within
+configuration within 1% on every seed; with roulette, the mean over 40 seeds within
 1% (tests/adjoint_fd.py);
 (3) first-order gradients (no roulette curvature term) against the oracle's fp64
 paths at 4x the samples, the loss adjoint taken from 256-sample images
@@ -117,20 +117,20 @@ def test_adjoint_matches_central_differences_c4_without_roulette(gpu):
 
 def test_adjoint_matches_central_differences_c4(gpu):
     # 512x512, 64 samples, depth 6 with Russian roulette from bounce 3: the mean over
-    # 24 seeds of the adjoint against the mean of central differences (h = 1e-3),
+    # 40 seeds of the adjoint against the mean of central differences (h = 1e-3),
     # within 1%. The roulette flips inside +-h make each seed's differences noisy
-    # (1-3% relative L2 per seed); the mean over 24 seeds brings that noise near
-    # 0.4%. Without the roulette curvature term the adjoint is biased by about 5.5%,
-    # which this test must resolve.
+    # (2-5% relative L2 per seed); the mean over 40 seeds brings that noise near
+    # 0.5%. Without the roulette curvature term the adjoint is biased by about 6%,
+    # which this test must resolve. About 2.6 s per seed on an MI355X.
     from tests.adjoint_fd import compare, seed_list
-    errs, g, fd, g1, each = compare(seed_list(24))
+    errs, g, fd, g1, each = compare(seed_list(40))
     first_order = np.linalg.norm(g1 - fd) / np.linalg.norm(fd)
-    print(f"C4: relative L2 error of the mean over 24 seeds {errs[-1]:.4f} (first-order {first_order:.4f}; "
+    print(f"C4: relative L2 error of the mean over 40 seeds {errs[-1]:.4f} (first-order {first_order:.4f}; "
           f"per seed {min(each):.4f}..{max(each):.4f})")
     assert errs[-1] <= 0.01
     assert first_order >= 2 * errs[-1]
     big = np.abs(fd) >= 0.2 * np.abs(fd).max()
-    np.testing.assert_allclose(g[big], fd[big], rtol=0.02)
+    np.testing.assert_allclose(g[big], fd[big], rtol=0.03)
 
 
 def test_adjoint_close_to_high_spp_fp64_reference(gpu):
