@@ -24,7 +24,10 @@
 namespace prt {
 namespace {
 
-constexpr int kBins = 32;
+#ifndef PRT_BVH_BINS
+#define PRT_BVH_BINS 32
+#endif
+constexpr int kBins = PRT_BVH_BINS;
 constexpr uint32_t kTaskGrain = 32768;
 
 struct Prim {
@@ -383,7 +386,10 @@ void build_bvh(const float* pos, uint32_t n, const BvhParams& params, BvhLayout&
     // children's padded boxes. The padding keeps box rejection conservative
     // with respect to the intersection test.
     std::vector<float> pb(size_t(total) * 6);
-    const float abs_pad = abs_max * 0x1p-16f;
+#ifndef PRT_BVH_PAD_ABS
+#define PRT_BVH_PAD_ABS 0x1p-16f
+#endif
+    const float abs_pad = abs_max * PRT_BVH_PAD_ABS;
     for (int64_t i = int64_t(total) - 1; i >= 0; --i) {
         const BNode& nd = bn[size_t(i)];
         float* b = &pb[size_t(i) * 6];

@@ -64,7 +64,14 @@ struct BvhParams {
     float tri_cost = PRT_BVH_TRI_COST;
     int merge_max = PRT_BVH_MERGE_MAX;
     int sah_depth_cap = 64;   // below this depth, object-median splits only
-    float pad_rel = 1.0f / 128.0f;  // leaf box padding relative to its extent
+// Leaf box padding relative to its extent (the absolute part, PRT_BVH_PAD_ABS of the
+// largest scene coordinate, is in bvh.cpp). Both keep box rejection conservative
+// with respect to fp32 Moller-Trumbore, whose accepted hits can lie slightly
+// outside the triangle for grazing rays from far away.
+#ifndef PRT_BVH_PAD_REL
+#define PRT_BVH_PAD_REL (1.0f / 128.0f)
+#endif
+    float pad_rel = PRT_BVH_PAD_REL;
 };
 
 // positions: n triangles x 3 vertices x xyz. abs_max: largest |coordinate| (>= 1).
