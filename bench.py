@@ -393,10 +393,11 @@ def main():
                                                 "same GPU: one wave64 VALU instruction per CU per cycle "
                                                 "(bench.py --issue-probe; profiles/)"}
     else:
-        # No counter summary for this configuration: the algorithmic rate stands in.
-        roofline.update(achieved=round(algorithmic, 1), frac=round(algorithmic / HBM_PEAK_GBS, 4),
-                        basis="algorithmic bytes per launch / live launch time (no counter summary for this "
-                              "configuration; caches serve much of it)")
+        # No counter summary for this configuration: HBM bytes are not measured, and the
+        # algorithmic rate (much of it served by the caches) is no fraction of the HBM peak.
+        roofline.update(achieved=None, frac=None,
+                        basis="no HBM counter summary for this configuration (profiles/ holds the default "
+                              "frame's); see algorithmic_GBs")
 
     out = {
         "metric": METRIC,
