@@ -418,6 +418,137 @@ __device__ __forceinline__ bool traverse(const SceneArgs& sc, V3 o, V3 d, float 
     return !ANY && (best.t != t0 || best.id != id0);
 }
 
+// ---- wave-wide (packet) traversal -----------------------------------------
+// For rays that start together and run close to each other (a wave's 64 camera
+// samples of one pixel or pixel block), the wave walks ONE depth-first path
+// through the tree: a child is entered when any live lane's ray hits its box, and
+// every live lane tests every triangle of every leaf entered. Node and triangle
+// indices are then the same in every lane, so their 64 / 48 bytes are read once
+// for the wave with scalar loads (scalar data cache, SGPR operands of the box and
+// triangle tests) and the stack is a single wave-wide LDS array. Each lane still
+// culls with its own [tmin, best.t] and accepts hits by the (t, original id) rule,
+// and the leaves entered are a superset of those its own traversal would reach, so
+// each lane's closest hit (or occlusion) is the one traverse() finds.
+template <class T>
+using kptr = const __attribute__((address_space(4))) T*;
+// p[i] with a wave-uniform i: one scalar load for the wave.
+template <class T>
+__device__ __forceinline__ T sload(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((kptr<T>)p)[i];
+#else
+    return p[i];
+#endif
+}
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
+
+// Entries of the wave-wide stack: at most three per level below the root, and
+// scenes are limited to 3 (depth + 1) <= kStackMax.
+constexpr int kPacketStack = kStackMax;
+constexpr uint32_t kPkNone = 0xFFFFFFFFu;  // sort key of a child no live lane enters
+
+// ANY = false: closest hit into best (t, rec, id, u, v, mat). ANY = true: occlusion
+// test against best (t, id), `occluded` set per lane. `live` lanes take part (a
+// lane with no ray, or an occluded one, tests nothing); nodes and tris must be the
+// same in every live lane. COUNT books each live lane's node and triangle tests.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ void packet_traverse(const float4* nodes, const float4* tris, bool live, const TravRay& r,
+                                                float tmin, HitRec& best, lds_int* stk, Counts& cnt, bool& occluded) {
+    occluded = false;
+    unsigned long long lm = __ballot(live);
+    if (lm == 0ull) return;
+    const float ox = r.o.x * r.ix, oy = r.o.y * r.iy, oz = r.o.z * r.iz;
+    const bool fx = r.ix < 0.f, fy = r.iy < 0.f, fz = r.iz < 0.f;  // the hi plane is the near one
+    const uint4* un = reinterpret_cast<const uint4*>(nodes);
+    int node = 0;
+    int sp = 0;
+    while (true) {
+        if (node >= 0) {  // an inner node
+            if (COUNT && live) cnt.nodes++;
+            const uint32_t nb = uint32_t(node) * 4u;
+            const float4 org = sload(nodes, nb);
+            const uint4 w = sload(un, nb + 1), qxy = sload(un, nb + 2), qz = sload(un, nb + 3);
+            const uint32_t ex = __float_as_uint(org.w);
+            const float sx = __uint_as_float((ex & 0xFFu) << 23), sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23),
+                        sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
+            const uint32_t nqx = fx ? qxy.y : qxy.x, fqx = fx ? qxy.x : qxy.y;
+            const uint32_t nqy = fy ? qxy.w : qxy.z, fqy = fy ? qxy.z : qxy.w;
+            const uint32_t nqz = fz ? qz.y : qz.x, fqz = fz ? qz.x : qz.y;
+            const int rep = __builtin_ctzll(lm);  // the first live lane orders the children
+            uint32_t key[4];
+            int word[4] = {int(w.x), int(w.y), int(w.z), int(w.w)};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float nx = __builtin_fmaf(qbyte(nqx, c), sx, org.x), fx_ = __builtin_fmaf(qbyte(fqx, c), sx, org.x);
+                const float ny = __builtin_fmaf(qbyte(nqy, c), sy, org.y), fy_ = __builtin_fmaf(qbyte(fqy, c), sy, org.y);
+                const float nz = __builtin_fmaf(qbyte(nqz, c), sz, org.z), fz_ = __builtin_fmaf(qbyte(fqz, c), sz, org.z);
+                const float tn = fmaxf(fmaxf(__builtin_fmaf(nx, r.ix, -ox), __builtin_fmaf(ny, r.iy, -oy)),
+                                       fmaxf(__builtin_fmaf(nz, r.iz, -oz), tmin));
+                const float tf = fminf(fminf(__builtin_fmaf(fx_, r.ix, -ox), __builtin_fmaf(fy_, r.iy, -oy)),
+                                       fminf(__builtin_fmaf(fz_, r.iz, -oz), best.t));
+                const bool hit = live && tn <= tf && word[c] != kNoChild;
+                // tn >= tmin > 0: the bit patterns of the keys order like the values.
+                const uint32_t mine = hit ? __float_as_uint(tn) : 0x7F800000u;
+                key[c] = __ballot(hit) != 0ull ? uint32_t(__builtin_amdgcn_readlane(int(mine), rep)) : kPkNone;
+            }
+            // Nearest first for the representative lane; children only other lanes
+            // enter follow, then children no lane enters (kPkNone).
+            auto cs = [](uint32_t& ka, int& wa, uint32_t& kb, int& wb) {
+                if (kb < ka) {
+                    const uint32_t k = ka;
+                    ka = kb;
+                    kb = k;
+                    const int t = wa;
+                    wa = wb;
+                    wb = t;
+                }
+            };
+            cs(key[0], word[0], key[1], word[1]);
+            cs(key[2], word[2], key[3], word[3]);
+            cs(key[0], word[0], key[2], word[2]);
+            cs(key[1], word[1], key[3], word[3]);
+            cs(key[1], word[1], key[2], word[2]);
+            if (key[3] != kPkNone) stk[sp++] = word[3];
+            if (key[2] != kPkNone) stk[sp++] = word[2];
+            if (key[1] != kPkNone) stk[sp++] = word[1];
+            if (key[0] != kPkNone) node = word[0];
+            else node = sp ? int(uniform_u32(uint32_t(stk[--sp]))) : kNoChild;
+        } else {  // a leaf
+            const uint32_t wl = uint32_t(node);
+            const uint32_t first = wl & 0x0FFFFFFFu;
+            const uint32_t count = ((wl >> 28) & 7u) + 1u;
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t tb = (first + k) * 3u;
+                const float4 a = sload(tris, tb), b = sload(tris, tb + 1), c = sload(tris, tb + 2);
+                if (COUNT && live) cnt.tris++;
+                float t, u, v;
+                if (live && mt_intersect(a, b, c, r.o, r.d, t, u, v)) {
+                    const uint32_t id = __float_as_uint(a.w);
+                    if (t > tmin && (t < best.t || (t == best.t && id < best.id))) {
+                        if (ANY) {
+                            occluded = true;
+                            live = false;
+                        } else {
+                            best.t = t;
+                            best.rec = first + k;
+                            best.id = id;
+                            best.u = u;
+                            best.v = v;
+                            best.mat = __float_as_uint(b.w);
+                        }
+                    }
+                }
+            }
+            if (ANY) {
+                lm = __ballot(live);
+                if (lm == 0ull) return;
+            }
+            node = sp ? int(uniform_u32(uint32_t(stk[--sp]))) : kNoChild;
+        }
+        if (node == kNoChild) return;
+    }
+}
+
 // u, v and material of best.rec, by the intersection test that selected it
 // (identical inputs, so identical results).
 __device__ __forceinline__ void hit_details(const float4* tris, const TravRay& r, HitRec& best) {
