@@ -15,8 +15,10 @@ Megakernel leg: the same frame with the one-kernel-per-launch integrator; its
 image must equal the wavefront's bit for bit (so the RMSE against any reference
 is the same), and the ratio of the two rates is reported.
 
-Roofline of the dominant kernel (wf_extend, closest-hit traversal of every primary
-and extension ray). The traversal is irregular pointer chasing with no dense
+Roofline of the dominant kernel: the extend launches (closest-hit traversal of
+every primary and extension ray; wf_extend_packet traces the first iteration's
+camera rays 64 at a time as packets, wf_extend the other iterations' rays one per
+lane). The traversal is irregular pointer chasing with no dense
 contraction, so the roofline is HBM's (peak 8 TB/s). `achieved` = measured HBM
 bytes per extend launch (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled
 by the same counter on a plain 1 GiB copy; read from the committed summary under
@@ -27,8 +29,9 @@ achieved / peak. Beside it: the kernel's algorithmic bytes per launch
     64 B x four-wide node visits + 48 B x triangle records tested
     + 48 B x hits (the winning record re-read for u, v, material)
     + 52 B x rays (32 B queue entry read, 20 B hit record written)
-(device counters of the instrumented frame, shadow rays removed), most of which
-the L2 and the MALL serve; and the instruction-issue fraction: the kernel's VALU
+(device counters of the instrumented frame, shadow rays removed; a packet's node
+and triangle tests count once per lane that makes them, although the packet reads
+each node and triangle once), most of which the L2 and the MALL serve; and the instruction-issue fraction: the kernel's VALU
 work per launch (committed SQ counter summary) per second of the
 live launch time, over the rate a calibration kernel of independent fp32 FMAs
 reaches on the same GPU (bench.py --issue-probe, committed summary), counted in
@@ -199,7 +202,7 @@ def summary_value(path: Path, key: str):
 
 
 def measured_traffic():
-    """HBM bytes per wf_extend launch from the committed rocprofv3 summary, or None."""
+    """HBM bytes per extend launch (mean of the frame's 8) from the committed rocprofv3 summary, or None."""
     return summary_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
 
 
@@ -354,7 +357,7 @@ def main():
         ext_bytes = (nb * (st.nodes_visited - st.anyhit_nodes) + 48 * (st.tris_tested - st.anyhit_tris)
                      + 48 * st.hits + 52 * ext_rays)
         bytes_per_launch = ext_bytes / launches
-        kernel = "wf_extend"
+        kernel = "extend launches (wf_extend_packet + wf_extend)"
         kernel_times = {"extend": round(sp.kernel_ms[0], 2), "shade": round(sp.kernel_ms[1], 2),
                         "shadow": round(sp.kernel_ms[2], 2), "launches_each": launches,
                         "profiled_frame_ms": round(sp.last_render_ms, 2)}
