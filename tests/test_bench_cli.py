@@ -1,6 +1,7 @@
-"""bench.py's command line, without a GPU: the one-GPU refusal and the per-workload
+"""bench.py's command line, without a GPU: the one-GPU refusal, the per-workload
 defaults (1080p x 128 spp for the 1M-triangle frame, 4K x 512 spp for the large
-scene, 256^2 x 16 spp at depth 5 for the Cornell fixture)."""
+scene, 256^2 x 16 spp at depth 5 for the Cornell fixture), the CPU baselines and
+the committed counter summaries it reads."""
 import subprocess
 import sys
 from pathlib import Path
@@ -73,3 +74,16 @@ def test_rmse_reference_fixture_and_metric():
     img.reshape(-1, 3)[ref["pixels"]] += np.float32(0.01)
     assert abs(bench.rmse_against_reference(img, Wl)["value"] - 0.01) < 1e-6
     assert W.SEED == int(ref["seed"])
+
+
+def test_batch_cpu_baseline_covers_every_view():
+    """The batched workload's CPU leg: the oracle on pixel-samples of all 64 views,
+    each with its own scene, camera and view id (a short sample here)."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    from cudart_amd import workloads as W
+
+    meshes, views, depth = W.batch()
+    r = bench.batch_cpu_baseline(meshes, views, depth, 0.2, 2)
+    assert r["kind"] == "port" and r["cores"] == 2 and r["value"] > 0
+    assert f"each of the {len(views)} views" in r["sample"]
