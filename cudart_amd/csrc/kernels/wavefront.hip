@@ -58,6 +58,11 @@ constexpr int kMaxIter = 64;
 constexpr uint32_t kSpread = PRT_FETCH_SPREAD;
 constexpr uint32_t kBatch = PRT_FETCH_BATCH;  // items per fetch atomic
 constexpr int kShadeBlock = PRT_SHADE_BLOCK;
+// Waves of wf_shade per compute unit in its grid (its occupancy limit, 4 waves per SIMD).
+#ifndef PRT_SHADE_GRID_WAVES
+#define PRT_SHADE_GRID_WAVES 16
+#endif
+constexpr int kShadeGridWaves = PRT_SHADE_GRID_WAVES;
 // Bytes of queues and path state per slot: 2 queues x 64, hit 16 + 4, shadow 64, result 16.
 constexpr size_t kSlotBytes = 2 * 64 + 16 + 4 + 64 + 16;
 // Counter words per 128-byte cache line. Each queue counter and each fetch counter
@@ -278,7 +283,9 @@ static_assert(PRT_SHADOW_PHASE_MIN >= 0 && PRT_SHADOW_PHASE_MIN <= 64, "phase th
 #define PRT_EXTEND_WAVES __attribute__((amdgpu_waves_per_eu(8)))
 #endif
 // wf_shade keeps the compiler's occupancy (104 VGPRs, 4 waves per SIMD): forcing 5
-// or 6 waves spills 36 / 104 bytes per lane and measured 2% / 12% slower per frame.
+// or 6 waves spills 36 / 104 bytes per lane and measured 2% / 12% slower per frame;
+// with 256-thread workgroups at 5 waves (24 bytes spilled) and 128-thread ones at 6,
+// each with a grid of 20 / 24 waves per compute unit, 4% and 25% slower.
 #ifndef PRT_SHADE_WAVES
 #define PRT_SHADE_WAVES
 #endif
@@ -1042,7 +1049,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
     hipDeviceProp_t prop;
     WF_TRY(hipGetDeviceProperties(&prop, device));
     const int gblocks = prop.multiProcessorCount * 8;
-    const int sblocks = prop.multiProcessorCount * 1024 / kShadeBlock;  // 16 waves of shade per compute unit
+    const int sblocks = prop.multiProcessorCount * kShadeGridWaves * 64 / kShadeBlock;
 
     if (profile) {
         rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, one_scene, rs);
