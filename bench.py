@@ -101,6 +101,8 @@ def parse():
                    help="with --frames-only: launch the wavefront chunks' kernels directly, not through graphs")
     p.add_argument("--copy-probe", action="store_true",
                    help="run a 1 GiB device-to-device copy 5 times and exit (counter calibration)")
+    p.add_argument("--l1-probe", action="store_true",
+                   help="run the vector-load gather calibration kernel and exit (L1 / L2 line-rate ceiling)")
     p.add_argument("--issue-probe", action="store_true",
                    help="run the fp32-FMA issue calibration kernel 5 times and exit (counter calibration)")
     a = p.parse_args()
@@ -145,6 +147,37 @@ def issue_probe():
     print(json.dumps({"issue_probe_fma_instructions_per_launch": fma, "launches": launches,
                       "ms_per_launch": round(ms.value / launches, 4),
                       "fma_instructions_per_s": fma / (ms.value / launches * 1e-3)}))
+
+
+def l1_probe():
+    """16-byte vector loads from pseudo-random 64-byte lines (4 independent loads per
+    line, 32 waves per compute unit): buffers of 16 KiB (L1), 1 MiB (L2) and 64 MiB;
+    the 64 lanes of a load on 64 lines, on one line, or 16 to a line. Rates are lane
+    loads per compute unit per ns (each lane's 16 bytes count once)."""
+    import ctypes as C
+
+    lib = C.CDLL(str(ROOT / "cudart_amd" / "lib" / "libprt_probe.so"))
+    lib.prt_probe_l1.restype = C.c_int
+    lib.prt_probe_l1.argtypes = [C.c_uint32] * 5 + [C.POINTER(C.c_float)]
+    lib.prt_probe_cu_count.restype = C.c_int
+    cus = lib.prt_probe_cu_count()
+    if cus <= 0:
+        raise RuntimeError(f"l1 probe: no device (hipError {-cus})")
+    blocks, iters, launches = cus * 8, 2000, 3
+    rows = []
+    for kib in (16, 1024, 65536):
+        for mode in (0, 1, 2):
+            ms = C.c_float()
+            rc = lib.prt_probe_l1(kib * 16, mode, blocks, iters, 1, C.byref(ms))  # warm-up
+            rc = rc or lib.prt_probe_l1(kib * 16, mode, blocks, iters, launches, C.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"l1 probe failed: {rc}")
+            instr = blocks * 4 * iters * 4 * launches  # wave64 load instructions
+            ns = ms.value * 1e6
+            rows.append({"buffer_kib": kib, "lanes_per_line": [1, 64, 16][mode], "ms": round(ms.value / launches, 4),
+                         "load_instructions_per_cu_per_ns": round(instr / cus / ns, 5),
+                         "lane_loads_per_cu_per_ns": round(instr * 64 / cus / ns, 4)})
+    print(json.dumps({"l1_probe": rows, "compute_units": cus}))
 
 
 def host_cores() -> int:
@@ -412,6 +445,9 @@ def main():
         return
     if a.issue_probe:
         issue_probe()
+        return
+    if a.l1_probe:
+        l1_probe()
         return
     if a.workload == "batch":
         batch_main(a)

@@ -1,8 +1,14 @@
-// Calibration kernel for instruction-issue measurements: the VALU issue rate a
-// kernel can reach on this GPU when nothing but independent fp32 FMAs are issued.
-// bench.py --issue-probe launches it; rocprofv3 counts its SQ_INSTS_VALU, and the
-// traversal kernels' VALU instructions per second are quoted as a fraction of
-// this kernel's. Not part of the renderer (built into its own libprt_probe.so).
+// Calibration kernels, not part of the renderer (built into their own
+// libprt_probe.so):
+//   valu_fma_probe   the VALU issue rate a kernel reaches on this GPU when nothing but
+//                    independent fp32 FMAs are issued (bench.py --issue-probe;
+//                    rocprofv3 counts its SQ_INSTS_VALU, and the traversal kernels'
+//                    VALU rates are quoted as a fraction of it);
+//   l1_gather_probe  the rate of 16-byte vector loads from 64-byte lines of a
+//                    buffer that fits the L1, the L2 or neither, with the 64 lanes of
+//                    each load on 64 different lines, on one line, or 16 to a line
+//                    (bench.py --l1-probe): the ceiling for the traversal kernels'
+//                    per-lane node and triangle fetches.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -27,7 +33,69 @@ __global__ __launch_bounds__(256) void valu_fma_probe(float* out, uint32_t iters
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Every thread loads the four 16-byte parts of `iters` pseudo-random 64-byte lines
+// of buf (line index masked to the buffer: always in bounds), each line's four loads
+// independent of the others. MODE 0: each lane its own line; 1: the wave's first
+// lane's line in every lane; 2: 16 lanes to a line (4 lines per load).
+template <int MODE>
+__global__ __launch_bounds__(256) void l1_gather_probe(const float4* buf, uint32_t mask_lines, uint32_t iters,
+                                                       float* out) {
+    uint32_t s = (blockIdx.x * 256u + threadIdx.x) * 2654435761u + 12345u;
+    float ax = 0.f, ay = 0.f;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t i = 0; i < iters; ++i) {
+        s = s * 1664525u + 1013904223u;
+        const uint32_t u = MODE == 0 ? s : uint32_t(__builtin_amdgcn_readfirstlane(int(s)));
+        const uint32_t line = ((u >> 8) + (MODE == 2 ? (lane >> 4) : 0u)) & mask_lines;
+        const float4* p = buf + size_t(line) * 4;
+        const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+        ax += a.x + b.y;
+        ay += c.z + d.w;
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = ax + ay;
+}
+
 }  // namespace
+
+// Runs l1_gather_probe (mode 0, 1 or 2) `launches` times over `blocks` workgroups of
+// 256 threads on a zeroed buffer of `lines` 64-byte lines (a power of two) and an
+// output of its own. *ms = the device time of all launches; 0, or a negative
+// hipError_t (-1 for a bad argument).
+extern "C" int prt_probe_l1(uint32_t lines, uint32_t mode, uint32_t blocks, uint32_t iters, uint32_t launches,
+                            float* ms) {
+    if (lines == 0 || (lines & (lines - 1)) != 0 || mode > 2 || !ms) return -1;
+    float4* buf = nullptr;
+    float* out = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&buf), size_t(lines) * 64);
+    if (e == hipSuccess) e = hipMemset(buf, 0, size_t(lines) * 64);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&out), size_t(blocks) * 256 * sizeof(float));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (uint32_t l = 0; l < launches && e == hipSuccess; ++l) {
+        if (mode == 0) hipLaunchKernelGGL(l1_gather_probe<0>, dim3(blocks), dim3(256), 0, nullptr, buf, lines - 1, iters, out);
+        if (mode == 1) hipLaunchKernelGGL(l1_gather_probe<1>, dim3(blocks), dim3(256), 0, nullptr, buf, lines - 1, iters, out);
+        if (mode == 2) hipLaunchKernelGGL(l1_gather_probe<2>, dim3(blocks), dim3(256), 0, nullptr, buf, lines - 1, iters, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (buf) (void)hipFree(buf);
+    if (out) (void)hipFree(out);
+    return e == hipSuccess ? 0 : -int(e);
+}
+
+// Compute units of the current device, or a negative hipError_t.
+extern "C" int prt_probe_cu_count(void) {
+    int dev = 0, n = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    return e == hipSuccess ? n : -int(e);
+}
 
 // Launches the probe `launches` times over `blocks` workgroups of 256 threads with
 // `iters` loop steps (kChains FMAs each), into an output buffer of its own. Returns
