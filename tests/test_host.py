@@ -28,6 +28,13 @@ def test_every_declared_entry_point_is_exported():
     assert ca.lib().prt_abi_version() == 1
 
 
+def test_calibration_probe_library_exports_its_kernels():
+    """bench.py --issue-probe / --l1-probe load the probe library (no GPU call here)."""
+    lib = C.CDLL(str(ROOT / "cudart_amd" / "lib" / "libprt_probe.so"))
+    for name in ("prt_probe_valu", "prt_probe_l1", "prt_probe_cu_count"):
+        assert hasattr(lib, name), name
+
+
 def test_cornell_obj_semantics():
     m = ca.Mesh.load_obj(W.ASSETS / "cornell.obj")
     a = m.arrays()
