@@ -25,6 +25,7 @@
 // Gradients accumulate in fp64: LDS per workgroup, then a fixed-order reduction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -43,13 +44,14 @@ struct AdjArgs {
     PathArgs path;
     uint32_t width, height, spp;
     uint32_t tiles_x, tiles;
+    uint32_t parts;       // workgroups per tile, each replaying a 1/parts range of its pixel-samples
     const float* grad;    // W*H*3 dLoss / dI
     const float4* samples;  // nullptr, or each sample's radiance from the forward pass [s * W * H + p]
     double norm;          // 1 / spp: dI / dL of one sample
     double half_hess;     // (d2 Loss / dI2) / (2 spp^2): loss curvature per sample radiance
     uint32_t num_params;  // 4 * materials: kd (M*3) then emission (M)
     uint32_t num_materials;
-    double* partial;      // grid x num_params
+    double* partial;      // grid x num_params (grid: tiles x parts rounded up to 8)
     int* spill;
     uint32_t spill_stride;
 };
@@ -320,9 +322,11 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
     __shared__ double s_g[4 * PRT_MAX_MATERIALS];
     for (uint32_t k = threadIdx.x; k < a.num_params; k += kBlock) s_g[k] = 0.0;
     __syncthreads();
-    const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
+    const uint32_t units = a.tiles * a.parts;
+    const uint32_t unit = xcd_tile(blockIdx.x, units);
     const int lane = int(threadIdx.x);
-    if (tile < a.tiles) {
+    if (unit < units) {
+        const uint32_t tile = unit / a.parts, part = unit - tile * a.parts;
         Stack st;
         st.bind(s_stack, a.spill + size_t(blockIdx.x) * kBlock, a.spill_stride);
         Counts cnt = {0, 0, 0, 0, 0, 0};
@@ -333,9 +337,11 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
         // lane per pixel and whole traversals in lockstep).
         const uint32_t tx = (tile % a.tiles_x) * 8, ty = (tile / a.tiles_x) * 8;
         const uint32_t cols = min(8u, a.width - tx), rows = min(8u, a.height - ty);
-        const uint32_t items = cols * rows * a.spp;
+        // This workgroup's share: items [i0, i1) of the tile's pixel-samples.
+        const uint64_t items = uint64_t(cols) * rows * a.spp;
+        const uint32_t i0 = uint32_t(items * part / a.parts), i1 = uint32_t(items * (part + 1) / a.parts);
         const unsigned long long below = (1ull << lane) - 1ull;
-        uint32_t next = 0;  // the first item not yet taken (the same in every lane)
+        uint32_t next = i0;  // the first item not yet taken (the same in every lane)
         int mode = kLaneIdle;
         Replay r;
         TravRay tr;
@@ -356,9 +362,9 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
             }
             const unsigned long long idle = __ballot(mode == kLaneIdle);
             const uint32_t nidle = uint32_t(__popcll(idle));
-            if ((nidle >= kAdjRefill || nidle == 64u) && next < items) {
+            if ((nidle >= kAdjRefill || nidle == 64u) && next < i1) {
                 const uint32_t j = next + uint32_t(__popcll(idle & below));
-                if (mode == kLaneIdle && j < items) {
+                if (mode == kLaneIdle && j < i1) {
                     const uint32_t p = j / a.spp, s = j - p * a.spp;
                     const int x = int(tx + p % cols), y = int(ty + p / cols);
                     const size_t pix = size_t(y) * a.width + size_t(x);
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
                 }
                 next += nidle;
             }
-            if (__ballot(mode != kLaneIdle) == 0ull && next >= items) break;
+            if (__ballot(mode != kLaneIdle) == 0ull && next >= i1) break;
             const bool busy = mode == kLaneExt || mode == kLaneShadow;
             const bool node_phase = node_phase_for<PRT_ADJOINT_PHASE_MIN>(busy && trav_wants_nodes(tr), busy);
             if (busy) {
@@ -463,6 +469,25 @@ bool primal_in_replay() {
 
 // The replay's arguments for the renderer's view: seed, view, depth and the scene
 // as the renderer holds them; grad = W*H*3 dLoss/dI on the device.
+// Workgroups per 8x8 tile of the replay. One workgroup per tile left most of the GPU
+// idle for the second half of the kernel: 512^2 x 64 samples of the 100k-triangle
+// scene is 4,096 tiles against 3,072 resident waves (3 per SIMD), and the tiles'
+// path costs differ, so the kernel ran 2.5 times as long as a tile's replay
+// (6.4 of 12 wave slots per compute unit occupied on average). Splitting each tile's
+// pixel-samples into `parts` ranges gives about 8 workgroups per resident wave slot.
+// The partials (ceil8(tiles x parts) x 4 x materials doubles) must also fit the
+// allocation-free entry point's scratch, so both entry points pick the same split and
+// give the same sums.
+uint32_t adjoint_parts(const prt_renderer* r, uint32_t spp) {
+    constexpr uint64_t kTargetUnits = 8ull * 3 * 4 * 256;  // 8 x (3 waves x 4 SIMDs x 256 CUs)
+    const uint64_t tiles = std::max<uint32_t>(1, r->tiles);
+    const uint64_t P = 4ull * r->scene->num_materials;
+    uint64_t parts = std::max<uint64_t>(1, (kTargetUnits + tiles - 1) / tiles);
+    parts = std::min<uint64_t>(parts, std::max<uint32_t>(1, spp));
+    while (parts > 1 && ((tiles * parts + 7) / 8) * 8 * P + P > kStaticScratchDoubles) --parts;
+    return uint32_t(parts);
+}
+
 AdjArgs adjoint_args(const prt_renderer* r, const float* grad, float hess, uint32_t spp, double* partial) {
     AdjArgs a;
     a.sc = r->sc;
@@ -473,6 +498,7 @@ AdjArgs adjoint_args(const prt_renderer* r, const float* grad, float hess, uint3
     a.spp = spp;
     a.tiles_x = r->tiles_x;
     a.tiles = r->tiles;
+    a.parts = adjoint_parts(r, spp);
     a.grad = grad;
     // The forward pass's per-sample radiance, when it was kept for exactly these
     // samples with the scene's current materials (same seed, view and depth: the
@@ -509,7 +535,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     const size_t px = size_t(r->w) * r->h;
     const uint32_t M = r->scene->num_materials;
     const uint32_t P = 4 * M;
-    const uint32_t grid = ((r->tiles + 7) / 8) * 8;
+    const uint32_t grid = ((r->tiles * adjoint_parts(r, spp) + 7) / 8) * 8;
     float* d_grad = nullptr;
     double* d_part = nullptr;
     double* d_out = nullptr;
@@ -587,7 +613,7 @@ extern "C" int prt_adjoint_grad_static(prt_renderer* r, const float* grad_rgb_de
     if (rc != PRT_OK) return rc;
     const uint32_t M = r->scene->num_materials;
     const uint32_t P = 4 * M;
-    const uint32_t grid = ((r->tiles + 7) / 8) * 8;
+    const uint32_t grid = ((r->tiles * adjoint_parts(r, spp) + 7) / 8) * 8;
     if (size_t(grid) * P + P > kStaticScratchDoubles)
         return fail(PRT_ERR_LIMIT, "image too large for the static scratch: (8x8 tiles rounded up to 8) x 4 x "
                                    "materials + 4 x materials must be at most 2^21 doubles");
