@@ -57,11 +57,15 @@ def _setup(w, h, spp, seed=1, max_depth=None, rr_start=3, render_seed=W.SEED):
     return wl, scene, r, render, kd0, em0, target, image
 
 
-def test_adjoint_equals_oracle_replay(gpu):
-    wl, scene, r, render, kd0, em0, target, image = _setup(64, 64, 4)
-    d_alb, d_em, loss = r.adjoint(target, 4, image=image)
+# 64x64 at 4 spp: 64 whole 8x8 tiles, each replayed by 4 workgroups of one sample;
+# 60x44 at 3 spp: partial tiles at the right and bottom edges, whose pixel-samples
+# split unevenly over 3 workgroups.
+@pytest.mark.parametrize("w,h,spp", [(64, 64, 4), (60, 44, 3)])
+def test_adjoint_equals_oracle_replay(gpu, w, h, spp):
+    wl, scene, r, render, kd0, em0, target, image = _setup(w, h, spp)
+    d_alb, d_em, loss = r.adjoint(target, spp, image=image)
     orc = O.OracleScene(wl.mesh.desc())
-    oa, oe = orc.adjoint(wl.camera, 64, 64, 4, O.cfg(max_depth=wl.max_depth, seed=W.SEED), image, target)
+    oa, oe = orc.adjoint(wl.camera, w, h, spp, O.cfg(max_depth=wl.max_depth, seed=W.SEED), image, target)
     g = np.concatenate([d_alb.ravel(), d_em]).astype(np.float64)
     og = np.concatenate([oa.ravel(), oe])
     scale = np.abs(og).max()
