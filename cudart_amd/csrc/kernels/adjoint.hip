@@ -296,23 +296,26 @@ __device__ int replay_hit(const AdjArgs& a, Replay& r, HitRec& h, bool hit, Pend
 #define PRT_ADJOINT_WAVES
 #endif
 // Idle lanes that make the wave start new pixel-samples. On 512^2 x 64 samples of
-// the 100k-triangle scene: 1 lane 71.7 / 73.1 ms, 8: 72.7 / 74.6, 16: 72.5 / 72.9,
-// 32: 79.8 / 79.0 (two runs each).
+// the 100k-triangle scene, one workgroup per tile: 1 lane 71.7 / 73.1 ms, 8: 72.7 /
+// 74.6, 16: 72.5 / 72.9, 32: 79.8 / 79.0 (two runs each); 6 workgroups per tile:
+// 1 lane 34.31 / 34.29 ms, 8: 34.04 / 34.11 (interleaved).
 #ifndef PRT_ADJOINT_REFILL
-#define PRT_ADJOINT_REFILL 1
+#define PRT_ADJOINT_REFILL 8
 #endif
 constexpr uint32_t kAdjRefill = PRT_ADJOINT_REFILL;
 // Lanes whose traversal has ended wait until this many have (or none is still
 // traversing), then do their vertices' work together (512^2 x 64 samples of the
 // 100k-triangle scene: 16 lanes 73.8 ms, 32: 66.3, 48: 68.8; each vertex's work at
-// the step its traversal ended: 77.9).
+// the step its traversal ended: 77.9; with 6 workgroups per tile 16: 37.7, 32: 34.3,
+// 48: 36.8).
 #ifndef PRT_ADJOINT_BATCH
 #define PRT_ADJOINT_BATCH 32
 #endif
 constexpr uint32_t kAdjBatch = PRT_ADJOINT_BATCH;
 // Lanes still wanting nodes below which the wave's step is a leaf phase. With the
 // lanes tracing independently most of them are mid-traversal at any step (512^2 x
-// 64 samples: 1 lane 66.7 ms, 8: 55.1, 12: 54.1, 16: 52.6, 20: 52.8, 24: 53.9).
+// 64 samples: 1 lane 66.7 ms, 8: 55.1, 12: 54.1, 16: 52.6, 20: 52.8, 24: 53.9; with
+// 6 workgroups per tile 8: 34.6, 16: 34.3, 24: 35.4).
 #ifndef PRT_ADJOINT_PHASE_MIN
 #define PRT_ADJOINT_PHASE_MIN 16
 #endif
