@@ -69,6 +69,9 @@ def _check_product_layout(mesh):
                 assert (lo[ids] > clo).all() and (hi[ids] < chi).all(), "leaf box is not padded"
                 covered[first:first + count] += 1
         assert used >= 1 and (used >= 2 or len(nodes) == 1), "an inner node holds at least two children"
+        inner = [int(w) for w in words[k] if 0 <= w < 0x7FFFFFFF]
+        if inner:
+            assert inner == list(range(inner[0], inner[0] + len(inner))), "siblings are consecutive"
         # the children's decoded boxes lie inside the parent's decoded box for this node
         assert (origin >= plo).all()
     assert (covered == 1).all(), "every record in exactly one leaf"
