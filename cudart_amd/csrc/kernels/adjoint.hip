@@ -291,9 +291,11 @@ __device__ int replay_hit(const AdjArgs& a, Replay& r, HitRec& h, bool hit, Pend
     return replay_after_nee(a, r, pd, false, tr, h, g, st);
 }
 
-// Occupancy of the replay kernel (empty: the compiler's choice).
+// Occupancy of the replay kernel: 3 waves per SIMD (168 VGPRs, two of them spilled).
+// Left to itself the compiler takes 172 VGPRs, 2 waves per SIMD: 45.2 against 34.0 ms
+// on 512^2 x 64 samples of the 100k-triangle scene.
 #ifndef PRT_ADJOINT_WAVES
-#define PRT_ADJOINT_WAVES
+#define PRT_ADJOINT_WAVES __attribute__((amdgpu_waves_per_eu(3)))
 #endif
 // Idle lanes that make the wave start new pixel-samples. On 512^2 x 64 samples of
 // the 100k-triangle scene, one workgroup per tile: 1 lane 71.7 / 73.1 ms, 8: 72.7 /

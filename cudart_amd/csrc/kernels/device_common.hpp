@@ -45,6 +45,8 @@ __device__ __forceinline__ float max_component(V3 v) {
 __device__ __forceinline__ V3 ld3(const float* p) { return V3{p[0], p[1], p[2]}; }
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) --------------------------------
+// Each round's two 32 x 32 -> 64-bit products are taken whole (one v_mad_u64_u32
+// each) rather than as separate low and high halves (v_mul_lo_u32 + v_mul_hi_u32).
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -52,9 +54,8 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-        uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c.x, p1 = uint64_t(0xCD9E8D57u) * c.z;
+        c = make_uint4(uint32_t(p1 >> 32) ^ c.y ^ k0, uint32_t(p1), uint32_t(p0 >> 32) ^ c.w ^ k1, uint32_t(p0));
     }
     return c;
 }
