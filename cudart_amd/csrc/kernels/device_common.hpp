@@ -203,10 +203,10 @@ __device__ __forceinline__ bool mt_intersect(float4 a, float4 b, float4 c, V3 o,
 }
 
 // One four-wide node (64 bytes, quantized child boxes): decodes each child plane as
-// fmaf(q, 2^e, origin) (exactly as the builder bounded it), slab-tests the four
-// boxes against [tmin, tfar], and orders the children hit nearest first. Pushes the
-// farther ones (farthest first, so the nearest of them is popped next) and returns
-// the nearest, or kNoChild.
+// fmaf(q, 2^e, origin) (exactly as the builder bounded it; with FOLD the decode is
+// folded into the slab FMA), slab-tests the four boxes against [tmin, tfar], and
+// orders the children hit nearest first. Pushes the farther ones (farthest first,
+// so the nearest of them is popped next) and returns the nearest, or kNoChild.
 //
 // The near and far plane of each axis are picked once per node from the sign of
 // the inverse direction, instead of a min and a max per child and axis: for i > 0,
