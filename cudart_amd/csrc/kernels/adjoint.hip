@@ -58,6 +58,13 @@ struct AdjArgs {
 
 namespace dev {
 
+// The replay's RNG: Philox with each product's low and high halves taken by
+// separate instructions. The 64-bit-product form takes this kernel from 167 to 172
+// VGPRs, 2 waves per SIMD instead of 3 (45.2 against 34.1 ms on 512^2 x 64 samples
+// of the 100k-triangle scene), or, held at 3 waves, spills two VGPRs to scratch
+// memory, which the allocation-free entry point should not need. Same bits.
+using AdjRng = RngT<false>;
+
 struct D3 {
     double x, y, z;
 };
@@ -92,7 +99,7 @@ __device__ __forceinline__ bool shadow_visible(const SceneArgs& sc, const NeeSam
 template <class StackType>
 __device__ V3 primal(const AdjArgs& a, int x, int y, uint32_t sample, StackType& st, Counts& cnt) {
     const SceneArgs& sc = a.sc;
-    Rng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
+    AdjRng rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
     V3 ro, rd;
     camera_ray(a.cam, x, y, rng.block(0), ro, rd);
     PathState ps;
@@ -140,7 +147,7 @@ struct VertexList {
 // while the others go on. L comes from the primal pass of the same sample.
 struct Replay {
     PathState ps;
-    Rng rng;
+    AdjRng rng;
     D3 rem, delta;
     V3 L;
     VertexList vl;
@@ -154,7 +161,7 @@ __device__ __forceinline__ void replay_begin(const AdjArgs& a, int x, int y, uin
     r.L = L;
     r.vl = VertexList();
     r.prev_ties = 0;
-    r.rng = Rng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
+    r.rng = AdjRng{a.path.seed, a.path.view, uint32_t(y) * uint32_t(a.cam.xpixels) + uint32_t(x), sample};
     V3 ro, rd;
     camera_ray(a.cam, x, y, r.rng.block(0), ro, rd);
     path_init(r.ps, ro, rd);
@@ -291,11 +298,9 @@ __device__ int replay_hit(const AdjArgs& a, Replay& r, HitRec& h, bool hit, Pend
     return replay_after_nee(a, r, pd, false, tr, h, g, st);
 }
 
-// Occupancy of the replay kernel: 3 waves per SIMD (168 VGPRs, two of them spilled).
-// Left to itself the compiler takes 172 VGPRs, 2 waves per SIMD: 45.2 against 34.0 ms
-// on 512^2 x 64 samples of the 100k-triangle scene.
+// Occupancy of the replay kernel (empty: the compiler's choice, 3 waves per SIMD).
 #ifndef PRT_ADJOINT_WAVES
-#define PRT_ADJOINT_WAVES __attribute__((amdgpu_waves_per_eu(3)))
+#define PRT_ADJOINT_WAVES
 #endif
 // Idle lanes that make the wave start new pixel-samples. On 512^2 x 64 samples of
 // the 100k-triangle scene, one workgroup per tile: 1 lane 71.7 / 73.1 ms, 8: 72.7 /

@@ -75,8 +75,8 @@ struct RrInfo {
 // load (latency hiding in the shade kernel); otherwise only the blocks the
 // material needs are evaluated (fewer registers in the megakernel). The values
 // are the same either way: a block depends only on (pixel, sample, bounce).
-template <bool COUNT, bool EAGER>
-__device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs& pa, const Rng& rng, PathState& ps,
+template <bool COUNT, bool EAGER, class RngType>
+__device__ __forceinline__ bool shade_vertex(const SceneArgs& sc, const PathArgs& pa, const RngType& rng, PathState& ps,
                                              const HitRec& h, bool* want_nee, NeeSample& ns, Counts& cnt,
                                              RrInfo* rr = nullptr) {
     *want_nee = false;
