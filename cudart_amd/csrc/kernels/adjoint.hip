@@ -391,8 +391,10 @@ __global__ __launch_bounds__(kBlock) PRT_ADJOINT_WAVES void adjoint_kernel(AdjAr
             const bool busy = mode == kLaneExt || mode == kLaneShadow;
             const bool node_phase = node_phase_for<PRT_ADJOINT_PHASE_MIN>(busy && trav_wants_nodes(tr), busy);
             if (busy) {
-                const int res = trav_step<false, false, false>(node_phase, a.sc.nodes, a.sc.tris, tr, kEps, h, st,
-                                                               cnt, mode == kLaneShadow);
+                // Both ray kinds take the folded box test here: 33.4 against 34.0 ms on
+                // 512^2 x 64 samples of the 100k-triangle scene.
+                const int res = trav_step<false, false, false, true>(node_phase, a.sc.nodes, a.sc.tris, tr, kEps, h,
+                                                                     st, cnt, mode == kLaneShadow);
                 if (res != kTravRunning) {
                     found = mode == kLaneExt ? (h.t != kFltMax || h.id != 0u) : res != kTravOccluded;
                     mode = mode == kLaneExt ? kLaneExtDone : kLaneShadowDone;

@@ -353,14 +353,16 @@ __device__ __forceinline__ bool trav_wants_nodes(const TravRay& r) { return r.le
 // written; hit_details() recomputes them from the record afterwards).
 // `any` (default ANY) may also be set per lane at run time, for kernels whose lanes
 // trace closest-hit and occlusion rays in the same steps.
-template <bool ANY, bool COUNT, bool LEAN, class StackType>
+// FOLD: the box test visit_node<FOLD> uses (by default the folded one for occlusion
+// rays only).
+template <bool ANY, bool COUNT, bool LEAN, bool FOLD = ANY, class StackType>
 __device__ __forceinline__ int trav_step(bool node_phase, const float4* nodes, const float4* tris, TravRay& r,
                                          float tmin, HitRec& best, StackType& st, Counts& cnt, bool any = ANY) {
     if (node_phase) {
         if (uint32_t(r.node) < uint32_t(kNoChild)) {  // an inner node
             if (COUNT) cnt.nodes++;
-            int next = visit_node<ANY>(nodes, r.node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz,
-                                       tmin, best.t, st);
+            int next = visit_node<FOLD>(nodes, r.node, r.ix, r.iy, r.iz, r.o.x * r.ix, r.o.y * r.iy, r.o.z * r.iz,
+                                        tmin, best.t, st);
             if (next == kNoChild && st.sp != 0) next = st.pop();
             r.node = next;
         }
