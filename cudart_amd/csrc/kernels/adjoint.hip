@@ -542,7 +542,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     if (r->cfg.max_depth > 16) return fail(PRT_ERR_LIMIT, "the adjoint supports max_depth <= 16");
     int rc = set_device(r->cfg.device);
     if (rc != PRT_OK) return rc;
-    rc = device_scene(r->scene, r->cfg.device, r->sc);
+    rc = device_scene(r->scene, r->cfg.device, r->sc, r->stream);
     if (rc != PRT_OK) return rc;
     const size_t px = size_t(r->w) * r->h;
     const uint32_t M = r->scene->num_materials;
@@ -560,7 +560,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_grad), px * 3 * sizeof(float)));
     ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_part), size_t(grid) * P * sizeof(double)));
     ADJ_TRY(hipMalloc(reinterpret_cast<void**>(&d_out), P * sizeof(double)));
-    ADJ_TRY(hipMemcpy(d_grad, grad_rgb, px * 3 * sizeof(float), hipMemcpyHostToDevice));
+    ADJ_TRY(copy_wait(d_grad, grad_rgb, px * 3 * sizeof(float), hipMemcpyHostToDevice, r->stream));
     a = adjoint_args(r, d_grad, hess, spp, d_part);
     // Without kept radiance, a wavefront forward pass first writes each sample's
     // radiance to a buffer of this call (when that fits in a quarter of the free
@@ -591,7 +591,7 @@ extern "C" int prt_adjoint_grad(prt_renderer* r, const float* grad_rgb, float he
     ADJ_TRY(hipEventRecord(e1, r->stream));
     ADJ_TRY(hipEventSynchronize(e1));
     ADJ_TRY(hipEventElapsedTime(&ms, e0, e1));
-    ADJ_TRY(hipMemcpy(g.data(), d_out, P * sizeof(double), hipMemcpyDeviceToHost));
+    ADJ_TRY(copy_wait(g.data(), d_out, P * sizeof(double), hipMemcpyDeviceToHost, r->stream));
     for (uint32_t k = 0; k < 3 * M; ++k) d_albedo[k] = float(g[k]);
     for (uint32_t k = 0; k < M; ++k) d_emission[k] = float(g[3 * M + k]);
     r->last_ms = ms;
@@ -621,7 +621,7 @@ extern "C" int prt_adjoint_grad_static(prt_renderer* r, const float* grad_rgb_de
     }
     // The renderer's creation uploaded the scene to this device: this refreshes at
     // most the material table in place (a copy, no allocation).
-    rc = device_scene(r->scene, r->cfg.device, r->sc);
+    rc = device_scene(r->scene, r->cfg.device, r->sc, r->stream);
     if (rc != PRT_OK) return rc;
     const uint32_t M = r->scene->num_materials;
     const uint32_t P = 4 * M;
@@ -647,7 +647,7 @@ extern "C" int prt_adjoint_grad_static(prt_renderer* r, const float* grad_rgb_de
     HIP_CHECK(hipEventRecord(r->ev1, r->stream));
     HIP_CHECK(hipEventSynchronize(r->ev1));
     HIP_CHECK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-    HIP_CHECK(hipMemcpy(g, d_out, P * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(copy_wait(g, d_out, P * sizeof(double), hipMemcpyDeviceToHost, r->stream));
     for (uint32_t k = 0; k < 3 * M; ++k) d_albedo[k] = float(g[k]);
     for (uint32_t k = 0; k < M; ++k) d_emission[k] = float(g[3 * M + k]);
     r->last_ms = ms;

@@ -40,18 +40,18 @@ void release_device_scene(DeviceScene& d) {
 
 namespace {
 template <class T>
-hipError_t upload(void** dst, const std::vector<T>& v) {
+hipError_t upload(void** dst, const std::vector<T>& v, hipStream_t stream) {
     size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
     hipError_t e = hipMalloc(dst, bytes);
     if (e != hipSuccess) return e;
-    if (!v.empty()) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (!v.empty()) e = copy_wait(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream);
     return e;
 }
 }  // namespace
 
 // Uploads the scene to `device` once (copyMemoryToCuda, parallel_renderer.cu:89-126)
 // and refreshes the material table when it changed.
-int device_scene(prt_scene* s, int device, SceneArgs& out) {
+int device_scene(prt_scene* s, int device, SceneArgs& out, hipStream_t stream) {
     std::lock_guard<std::mutex> g(s->dev_mutex);
     DeviceScene* d = nullptr;
     for (auto& x : s->devices)
@@ -60,15 +60,15 @@ int device_scene(prt_scene* s, int device, SceneArgs& out) {
         DeviceScene n;
         n.device = device;
         hipError_t e = hipSuccess;
-        if (e == hipSuccess) e = upload(&n.nodes, s->bvh.nodes);
-        if (e == hipSuccess) e = upload(&n.tris, s->tris);
-        if (e == hipSuccess) e = upload(&n.shade, s->shade);
-        if (e == hipSuccess) e = upload(&n.mats, s->mats);
-        if (e == hipSuccess) e = upload(&n.lights, s->lights);
-        if (e == hipSuccess) e = upload(&n.texels, s->texels);
-        if (e == hipSuccess) e = upload(&n.texinfo, s->texinfo);
+        if (e == hipSuccess) e = upload(&n.nodes, s->bvh.nodes, stream);
+        if (e == hipSuccess) e = upload(&n.tris, s->tris, stream);
+        if (e == hipSuccess) e = upload(&n.shade, s->shade, stream);
+        if (e == hipSuccess) e = upload(&n.mats, s->mats, stream);
+        if (e == hipSuccess) e = upload(&n.lights, s->lights, stream);
+        if (e == hipSuccess) e = upload(&n.texels, s->texels, stream);
+        if (e == hipSuccess) e = upload(&n.texinfo, s->texinfo, stream);
         if (e == hipSuccess) e = hipMalloc(&n.lin, 256 * sizeof(float));
-        if (e == hipSuccess) e = hipMemcpy(n.lin, s->lin, 256 * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_wait(n.lin, s->lin, 256 * sizeof(float), hipMemcpyHostToDevice, stream);
         n.mats_version = s->mats_version;
         if (e != hipSuccess) {
             release_device_scene(n);
@@ -77,7 +77,7 @@ int device_scene(prt_scene* s, int device, SceneArgs& out) {
         s->devices.push_back(n);
         d = &s->devices.back();
     } else if (d->mats_version != s->mats_version) {
-        HIP_TRY(hipMemcpy(d->mats, s->mats.data(), s->mats.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(copy_wait(d->mats, s->mats.data(), s->mats.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         d->mats_version = s->mats_version;
     }
     out.nodes = static_cast<const float4*>(d->nodes);
@@ -192,16 +192,17 @@ extern "C" int prt_renderer_create(prt_scene* scene, const prt_camera* cam, uint
     int rc = set_device(c.device);
     if (rc != PRT_OK) return rc;
     PRT_TRY_BEGIN
-    std::unique_ptr<prt_renderer> r(new prt_renderer);
+    // A failure part-way releases whatever was set up (stream, buffers).
+    std::unique_ptr<prt_renderer, int (*)(prt_renderer*)> r(new prt_renderer, prt_renderer_destroy);
     r->scene = scene;
     r->cam = cm;
     r->w = w;
     r->h = h;
     r->spp = spp;
     r->cfg = c;
-    rc = device_scene(scene, c.device, r->sc);
-    if (rc != PRT_OK) return rc;
     HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    rc = device_scene(scene, c.device, r->sc, r->stream);
+    if (rc != PRT_OK) return rc;
     HIP_TRY(hipEventCreate(&r->ev0));
     HIP_TRY(hipEventCreate(&r->ev1));
     const size_t px = size_t(w) * h;
@@ -212,7 +213,7 @@ extern "C" int prt_renderer_create(prt_scene* scene, const prt_camera* cam, uint
     uint8_t lut[256];
     gamma_table(lut);
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&r->gamma_lut), 256));
-    HIP_TRY(hipMemcpy(r->gamma_lut, lut, 256, hipMemcpyHostToDevice));
+    HIP_TRY(copy_wait(r->gamma_lut, lut, 256, hipMemcpyHostToDevice, r->stream));
     if (c.keep_samples && spp > 0)
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&r->samples), px * spp * sizeof(float4)));
     r->tiles_x = (w + 7) / 8;
@@ -231,7 +232,7 @@ extern "C" int prt_render_spp(prt_renderer* r, uint32_t n) {
     if (n == 0) return PRT_OK;
     int rc = set_device(r->cfg.device);
     if (rc != PRT_OK) return rc;
-    rc = device_scene(r->scene, r->cfg.device, r->sc);
+    rc = device_scene(r->scene, r->cfg.device, r->sc, r->stream);
     if (rc != PRT_OK) return rc;
     if (r->samples) {
         if (r->samples_done == 0) {
@@ -311,7 +312,7 @@ extern "C" int prt_get_radiance(prt_renderer* r, float* rgb) {
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tmp), bytes));
     rc = mean_to(r, tmp);
     if (rc == PRT_OK) {
-        hipError_t e = hipMemcpy(rgb, tmp, bytes, hipMemcpyDeviceToHost);
+        hipError_t e = copy_wait(rgb, tmp, bytes, hipMemcpyDeviceToHost, r->stream);
         if (e != hipSuccess) rc = hip_fail(e, "copying radiance");
     }
     (void)hipFree(tmp);
@@ -337,7 +338,7 @@ extern "C" int prt_copy_image_bytes(prt_renderer* r, uint8_t* rgba) {
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tmp), n * 4));
     hipError_t e = launch_tonemap(r->accum, uint32_t(n), 0.f, r->samples_done, r->gamma_lut, tmp, r->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-    if (e == hipSuccess) e = hipMemcpy(rgba, tmp, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_wait(rgba, tmp, n * 4, hipMemcpyDeviceToHost, r->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(e, "tonemap");
     return PRT_OK;
@@ -367,9 +368,9 @@ extern "C" int prt_first_hit(prt_renderer* r, uint32_t sample, int32_t* tri, int
     a.spill_stride = r->spill_threads;
     hipError_t e = launch_first_hit(a, r->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-    if (e == hipSuccess && tri) e = hipMemcpy(tri, a.tri, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && mat) e = hipMemcpy(mat, a.mat, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && t) e = hipMemcpy(t, a.t, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tri) e = copy_wait(tri, a.tri, n * 4, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && mat) e = copy_wait(mat, a.mat, n * 4, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && t) e = copy_wait(t, a.t, n * 4, hipMemcpyDeviceToHost, r->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(e, "first hit");
     return PRT_OK;
@@ -381,7 +382,7 @@ extern "C" int prt_get_stats(prt_renderer* r, prt_stats* st) {
     int rc = set_device(r->cfg.device);
     if (rc != PRT_OK) return rc;
     unsigned long long c[kCntNum];
-    HIP_TRY(hipMemcpy(c, r->counters, sizeof c, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_wait(c, r->counters, sizeof c, hipMemcpyDeviceToHost, r->stream));
     st->primary_rays = c[kCntPrimary];
     st->extension_rays = c[kCntExtension];
     st->shadow_rays = c[kCntShadow];
@@ -404,7 +405,7 @@ extern "C" int prt_reset_stats(prt_renderer* r) {
     if (!r) return fail(PRT_ERR_INVALID_ARG, "null argument");
     int rc = set_device(r->cfg.device);
     if (rc != PRT_OK) return rc;
-    HIP_TRY(hipMemset(r->counters, 0, kCntNum * sizeof(unsigned long long)));
+    HIP_TRY(fill_wait(r->counters, 0, kCntNum * sizeof(unsigned long long), r->stream));
     r->samples_total = 0;
     return PRT_OK;
 }

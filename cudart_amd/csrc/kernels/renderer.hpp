@@ -42,7 +42,20 @@ struct prt_renderer {
 
 namespace prt {
 int hip_fail(hipError_t e, const char* what);
-int device_scene(prt_scene* s, int device, SceneArgs& out);
+// Copies and fills ordered on the caller's stream and waited for. The library never
+// uses the legacy (null) stream: HIP fails a legacy-stream operation issued while
+// another thread captures a stream into a graph, and ends that capture, so renderers
+// driven from several host threads at once would break each other's launches.
+inline hipError_t copy_wait(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+inline hipError_t fill_wait(void* dst, int value, size_t bytes, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+// Uploads the scene to `device` on first use (copies ordered on `stream`).
+int device_scene(prt_scene* s, int device, SceneArgs& out, hipStream_t stream);
 CamArgs cam_args(const prt_camera& c);
 int set_device(int device);
 int ensure_spill(prt_renderer* r, uint32_t threads);

@@ -860,7 +860,7 @@ int ensure_buffer(T*& p, size_t& cap, size_t n) {
     return PRT_OK;
 }
 
-int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t persistent) {
+int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t persistent, hipStream_t stream) {
     if (w.mem && w.capacity >= capacity && w.persistent >= persistent && w.device == device) return PRT_OK;
     if (w.mem) (void)hipFree(w.mem);
     if (w.spill) (void)hipFree(w.spill);
@@ -872,7 +872,7 @@ int ensure_state(WavefrontState& w, int device, uint32_t capacity, uint32_t pers
     w.bytes = kSlotBytes * size_t(capacity);
     WF_TRY(hipMalloc(&w.mem, w.bytes));
     WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.ctr), kNumCounters * sizeof(uint32_t)));
-    WF_TRY(hipMemset(w.ctr, 0, kNumCounters * sizeof(uint32_t)));
+    WF_TRY(fill_wait(w.ctr, 0, kNumCounters * sizeof(uint32_t), stream));
     const size_t spill_entries = size_t(kStackMax - kStackLdsQueue) * persistent;
     WF_TRY(hipMalloc(reinterpret_cast<void**>(&w.spill), spill_entries * sizeof(int)));
     char* p = static_cast<char*>(w.mem);
@@ -1028,7 +1028,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
                              : (one_scene ? reinterpret_cast<const void*>(dev::wf_extend<false, true>)
                                           : reinterpret_cast<const void*>(dev::wf_extend<false, false>));
     const int pblocks = persistent_blocks(kext, blocks_per_cu, device);
-    int rc = ensure_state(w, device, need, uint32_t(pblocks) * kBlock);
+    int rc = ensure_state(w, device, need, uint32_t(pblocks) * kBlock, stream);
     if (rc != PRT_OK) return rc;
     rc = ensure_buffer(w.d_jobs, w.cap_jobs, jobs.size());
     if (rc == PRT_OK) rc = ensure_buffer(w.d_tiles, w.cap_tiles, tiles.size());
@@ -1053,7 +1053,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
 
     if (profile) {
         rc = run_profiled(a, stream, chunks.size(), max_depth, pblocks, gblocks, sblocks, count, one_scene, rs);
-        if (rc == PRT_OK && rs) WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
+        if (rc == PRT_OK && rs) WF_TRY(copy_wait(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost, stream));
         return rc;
     }
     // Capture one chunk's launch sequence.
@@ -1088,7 +1088,7 @@ int run_wavefront(WavefrontState& w, int device, hipStream_t stream, const std::
             rs->kernel_ms[k] = 0.0;
             rs->launches[k] = 0;
         }
-        WF_TRY(hipMemcpy(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost));
+        WF_TRY(copy_wait(rs->c, w.ctr + kStats, sizeof rs->c, hipMemcpyDeviceToHost, stream));
     }
     return PRT_OK;
 }
@@ -1130,9 +1130,9 @@ int render_wavefront(prt_renderer* r, uint32_t n) {
     if (rc != PRT_OK) return rc;
     // Fold the wavefront's counters into the renderer's.
     unsigned long long host[kCntNum];
-    WF_TRY(hipMemcpy(host, r->counters, sizeof host, hipMemcpyDeviceToHost));
+    WF_TRY(copy_wait(host, r->counters, sizeof host, hipMemcpyDeviceToHost, r->stream));
     for (int k = 0; k < kCntNum; ++k) host[k] += rs.c[k];
-    WF_TRY(hipMemcpy(r->counters, host, sizeof host, hipMemcpyHostToDevice));
+    WF_TRY(copy_wait(r->counters, host, sizeof host, hipMemcpyHostToDevice, r->stream));
     r->last_ms = rs.ms;
     for (int k = 0; k < 3; ++k) {
         r->kernel_ms[k] = rs.kernel_ms[k];
@@ -1184,7 +1184,7 @@ struct BatchCtx {
     uint64_t total = 0;
 };
 
-int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCtx& b) {
+int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCtx& b, hipStream_t stream) {
     if (!jobs || n == 0) return fail(PRT_ERR_INVALID_ARG, "no jobs");
     b.max_depth = c.max_depth;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1196,7 +1196,7 @@ int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCt
         if (uint64_t(jb.camera.xpixels) * uint64_t(jb.camera.ypixels) > (1u << 24))
             return fail(PRT_ERR_LIMIT, "a view with more than 2^24 pixels");
         WfJob w;
-        int rc = device_scene(jb.scene, c.device, w.sc);
+        int rc = device_scene(jb.scene, c.device, w.sc, stream);
         if (rc != PRT_OK) return rc;
         w.cam = cam_args(jb.camera);
         w.path = PathArgs{c.seed, jb.view_id, c.max_depth, c.rr_start};
@@ -1206,7 +1206,7 @@ int batch_prepare(prt_job* jobs, uint32_t n, const prt_render_config& c, BatchCt
         float4* acc = nullptr;
         WF_TRY(hipMalloc(reinterpret_cast<void**>(&acc), px * sizeof(float4)));
         b.accum.push_back(acc);
-        WF_TRY(hipMemset(acc, 0, px * sizeof(float4)));
+        WF_TRY(fill_wait(acc, 0, px * sizeof(float4), stream));
         w.accum = acc;
         w.samples = nullptr;
         w.samples_cap = 0;
@@ -1241,16 +1241,14 @@ extern "C" int prt_render_batch(prt_job* jobs, uint32_t n, const prt_render_conf
     if (rc != PRT_OK) return rc;
     PRT_TRY_BEGIN
     BatchCtx b;
-    rc = batch_prepare(jobs, n, c, b);
-    if (rc != PRT_OK) {
-        batch_release(b);
-        return rc;
-    }
     hipStream_t s;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    if (e != hipSuccess) {
+    if (e != hipSuccess) return hip_fail(e, "stream");
+    rc = batch_prepare(jobs, n, c, b, s);
+    if (rc != PRT_OK) {
         batch_release(b);
-        return hip_fail(e, "stream");
+        (void)hipStreamDestroy(s);
+        return rc;
     }
     WavefrontState w;
     const uint32_t cap = tune && tune->queue_slots ? tune->queue_slots : default_capacity(b.total);
@@ -1265,7 +1263,7 @@ extern "C" int prt_render_batch(prt_job* jobs, uint32_t n, const prt_render_conf
             e = hipMalloc(reinterpret_cast<void**>(&tmp), px * 3 * sizeof(float));
             if (e == hipSuccess) e = launch_mean(b.accum[i], uint32_t(px), jobs[i].spp, tmp, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e == hipSuccess) e = hipMemcpy(jobs[i].out_rgb, tmp, px * 3 * sizeof(float), hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = copy_wait(jobs[i].out_rgb, tmp, px * 3 * sizeof(float), hipMemcpyDeviceToHost, s);
             if (tmp) (void)hipFree(tmp);
             if (e != hipSuccess) rc = hip_fail(e, "batch readback");
         }
@@ -1312,15 +1310,16 @@ extern "C" int prt_autotune(prt_job* jobs, uint32_t n, const prt_render_config* 
         paths += uint64_t(t.camera.xpixels) * t.camera.ypixels * t.spp;
     }
     BatchCtx b;
-    rc = batch_prepare(trial.data(), n, c, b);
+    hipStream_t s;
+    WF_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    rc = batch_prepare(trial.data(), n, c, b, s);
     if (rc != PRT_OK) {
         batch_release(b);
+        (void)hipStreamDestroy(s);
         return rc;
     }
     uint64_t full = 0;
     for (uint32_t i = 0; i < n; ++i) full += uint64_t(jobs[i].camera.xpixels) * jobs[i].camera.ypixels * jobs[i].spp;
-    hipStream_t s;
-    WF_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     WavefrontState w;
     double bms = 1e30;
     prt_tune bt{};
