@@ -14,7 +14,7 @@ from cudart_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-CALLS, SPP = 6, 2
+CALLS, SPP = 24, 1
 
 
 def _renderers(scene, wl):
@@ -23,9 +23,12 @@ def _renderers(scene, wl):
 
 
 def _work(r, errors):
+    """Renders, reading the image back after each call (a readback's temporary
+    device buffer is allocated and freed while the other thread may be capturing)."""
     try:
         for _ in range(CALLS):
             r.render(SPP)
+            r.radiance()
     except Exception as e:  # reported by the main thread
         errors.append(e)
 
