@@ -56,3 +56,32 @@ def test_two_threads_render_as_one_after_the_other(gpu):
     for a, b in zip(want, got):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert not np.array_equal(want[0], want[1])  # the two views differ (seeds)
+
+
+def test_two_threads_batch_as_one_after_the_other(gpu):
+    """Two prt_render_batch calls at once (each on a stream of its own, each
+    capturing its chunk's launches) give the images of the same calls made alone."""
+    a = W.cornell(40, 30)
+    b = W.spheres1m(50, 34, instances=12)
+    sa, sb = ca.Scene(a.mesh), ca.Scene(b.mesh)
+    batches = [[ca.BatchJob(sa, a.camera, 3, view_id=1), ca.BatchJob(sb, b.camera, 2, view_id=2)],
+               [ca.BatchJob(sb, b.camera, 3, view_id=3), ca.BatchJob(sa, a.camera, 2, view_id=4)]]
+    want = [ca.render_batch(jobs, max_depth=6)[0] for jobs in batches]
+    got, errors = [None, None], []
+
+    def run(k):
+        try:
+            for _ in range(8):
+                got[k] = ca.render_batch(batches[k], max_depth=6)[0]
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for w, g in zip(want, got):
+        for x, y in zip(w, g):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
