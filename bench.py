@@ -151,9 +151,12 @@ def issue_probe():
 
 def l1_probe():
     """16-byte vector loads from pseudo-random 64-byte lines (4 independent loads per
-    line, 32 waves per compute unit): buffers of 16 KiB (L1), 1 MiB (L2) and 64 MiB;
-    the 64 lanes of a load on 64 lines, on one line, or 16 to a line. Rates are lane
-    loads per compute unit per ns (each lane's 16 bytes count once)."""
+    lane and step, 32 waves per compute unit): buffers of 16 KiB (L1), 1 MiB (L2) and
+    64 MiB; the 64 lanes of a load on 64 lines (each lane the four parts of its own
+    line), all on one line, 16 to a line, 4 to a line (the four lanes of a quad on the
+    four parts of one line), or 64 lines with every other lane masked off. Rates are
+    load instructions and active lane loads per compute unit per ns (each lane's 16
+    bytes count once)."""
     import ctypes as C
 
     lib = C.CDLL(str(ROOT / "cudart_amd" / "lib" / "libprt_probe.so"))
@@ -165,18 +168,21 @@ def l1_probe():
         raise RuntimeError(f"l1 probe: no device (hipError {-cus})")
     blocks, iters, launches = cus * 8, 2000, 3
     rows = []
+    layouts = ["64 lines", "1 line", "16 lanes per line", "4 lanes per line (quad, one part each)",
+               "64 lines, every other lane masked"]
     for kib in (16, 1024, 65536):
-        for mode in (0, 1, 2):
+        for mode in range(5):
             ms = C.c_float()
             rc = lib.prt_probe_l1(kib * 16, mode, blocks, iters, 1, C.byref(ms))  # warm-up
             rc = rc or lib.prt_probe_l1(kib * 16, mode, blocks, iters, launches, C.byref(ms))
             if rc != 0:
                 raise RuntimeError(f"l1 probe failed: {rc}")
             instr = blocks * 4 * iters * 4 * launches  # wave64 load instructions
+            lanes = 32 if mode == 4 else 64
             ns = ms.value * 1e6
-            rows.append({"buffer_kib": kib, "lanes_per_line": [1, 64, 16][mode], "ms": round(ms.value / launches, 4),
+            rows.append({"buffer_kib": kib, "mode": mode, "layout": layouts[mode], "ms": round(ms.value / launches, 4),
                          "load_instructions_per_cu_per_ns": round(instr / cus / ns, 5),
-                         "lane_loads_per_cu_per_ns": round(instr * 64 / cus / ns, 4)})
+                         "lane_loads_per_cu_per_ns": round(instr * lanes / cus / ns, 4)})
     print(json.dumps({"l1_probe": rows, "compute_units": cus}))
 
 
